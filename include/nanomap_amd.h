@@ -1,0 +1,183 @@
+/* nanomap_amd.h -- C ABI of the MI355X-native NanoMap frustum-sensor path.
+ *
+ * This is the drop-in boundary.  The reference drives its GPU path through
+ * three extern "C" launchers that take C++ objects by reference:
+ *
+ *   void filterCloud(nanomap::gpu::SensorBucket&, cudaStream_t)
+ *        -- src/nanomap/kernels/filterCloud.cu:476, declared
+ *           include/nanomap/handler/Handler.h:46
+ *   void frustumCastCloud(nanomap::gpu::SensorBucket&, cudaStream_t,
+ *                         cudaStream_t)
+ *        -- src/nanomap/kernels/frustumCastCloud.cu:311, Handler.h:49
+ *   void generateCloud(nanomap::gpu::SensorBucket&,
+ *                      nanovdb::GridHandle<CudaDeviceBuffer>&, cudaStream_t)
+ *        -- src/nanomap/kernels/generateCloud.cu:84, SimHandler.h:48
+ *
+ * and folds the result into the map on the host
+ * (Handler::voxelUpdateFromFrustumBuffer, src/nanomap/handler/Handler.cpp:134).
+ * Here every entry point takes plain pointers, sizes and POD structs, and
+ * returns an int status.  The map stays resident in device memory.
+ *
+ * Conventions
+ *  - Every function returns NM_OK (0) or an NM_E* code; nm_last_error()
+ *    returns a message for the calling thread's last failure.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *    Calls enqueue work and return; nothing blocks unless documented.
+ *  - A context owns its device buffers and serves one stream at a time;
+ *    distinct contexts may be used from distinct threads.
+ *  - Coordinates are voxel indices (index space = world metres / voxel
+ *    size).  Leaves are 8^3 voxels with origins at multiples of 8; the
+ *    voxel order inside a leaf is z + 8*y + 64*x (frustumCastCloud.cu:215).
+ *  - rounding: NM_ROUND_DOWN reproduces the reference host path, which runs
+ *    with FE_DOWNWARD left set (include/nanomap/sensor/SensorData.h:42);
+ *    NM_ROUND_NEAREST is IEEE default rounding.
+ */
+#ifndef NANOMAP_AMD_H
+#define NANOMAP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NM_OK 0
+#define NM_EINVAL 1     /* bad argument                               */
+#define NM_ENOMEM 2     /* allocation failed                          */
+#define NM_EOVERFLOW 3  /* a capacity was exceeded; result incomplete */
+#define NM_EHIP 4       /* HIP runtime error                          */
+#define NM_ENODEV 5     /* no usable gfx950 device                    */
+#define NM_ESTATE 6     /* call out of order (e.g. update w/o cast)   */
+
+#define NM_ROUND_NEAREST 0
+#define NM_ROUND_DOWN 1
+
+#define NM_FILTER_NONE 0  /* every point is a ray: Handler::populateTempGrid */
+#define NM_FILTER_VOXEL 1 /* voxel-centroid filter: filterCloud.cu:84,279   */
+
+typedef struct nm_ctx nm_ctx; /* per-sensor device workspace             */
+typedef struct nm_map nm_map; /* device-resident sparse log-odds map     */
+
+/* Context sizing.  Replaces SensorAllocator's buffer sizing
+ * (include/nanomap/allocator/SensorAllocator.h:33-108). */
+typedef struct nm_ctx_config {
+  int device;          /* HIP device ordinal                              */
+  int max_points;      /* largest cloud (pixels) per frame                */
+  float max_time;      /* largest max_range / voxel size, in voxels       */
+  int tile_capacity;   /* leaf tiles per frame; 0 = the worst case        */
+} nm_ctx_config;
+
+/* Per-frame parameters, precomputed on the host.  Replaces the device copy
+ * of gpu::Sensor + gpu::GridData (include/nanomap/gpu/Sensor.h:156-211,
+ * GridData.h:38-62; uploaded at SensorAllocator.h:120-121). */
+typedef struct nm_frame {
+  float rotation[9];    /* row-major R = normalized(q) * F (Handler.cpp:41) */
+  double eye[3];        /* sensor origin in index space (Handler.cpp:34)    */
+  float grid_res;       /* voxel size, metres                               */
+  float max_time;       /* max range / grid_res (Handler.cpp:39)            */
+  float logodds_hit;    /* added on the end voxel of a ray                  */
+  float logodds_miss;   /* added on every voxel a ray passes                */
+  int rounding;         /* NM_ROUND_*                                       */
+  int filter;           /* NM_FILTER_*                                      */
+  int width, height;    /* cloud layout (pcl_width, pcl_height)             */
+} nm_frame;
+
+/* Counters of the most recent frame of a context. */
+typedef struct nm_frame_stats {
+  int64_t points;        /* input points                                   */
+  int64_t rays;          /* rays cast (valid points or filter bins)        */
+  int64_t visits;        /* voxel updates (misses + hits)                  */
+  int64_t voxels;        /* unique voxels touched                          */
+  int64_t leaves;        /* 8^3 leaves touched                             */
+  int64_t hit_voxels;    /* voxels with at least one hit                   */
+  int64_t mixed_voxels;  /* voxels with both hits and misses               */
+  int64_t map_leaves;    /* leaves in the map after the update             */
+  int32_t errors;        /* internal consistency errors (0 expected)       */
+  int32_t overflow;      /* nonzero if a capacity was exceeded             */
+} nm_frame_stats;
+
+const char* nm_last_error(void);
+int nm_version(void);
+
+/* ---- context ---------------------------------------------------------- */
+int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out);
+int nm_ctx_destroy(nm_ctx* ctx);
+
+/* ---- map: map::Map (include/nanomap/map/OccupancyMap.h:35-61) ---------- */
+/* clamp_min/clamp_max: emptyClampingThreshold/occupiedClampingThreshold;
+ * hit_thr/miss_thr: logOddsHitThreshold/logOddsMissThreshold. */
+int nm_map_create(int device, float grid_res, float clamp_min,
+                  float clamp_max, float hit_thr, float miss_thr,
+                  int leaf_capacity, nm_map** out);
+int nm_map_destroy(nm_map* map);
+int nm_map_clear(nm_map* map, void* stream);
+/* Number of allocated leaves (synchronizes the map's last stream). */
+int nm_map_leaf_count(nm_map* map, int64_t* count);
+/* Copy all leaves to host arrays sized leaf_count: origins (3 ints/leaf),
+ * values (512 floats/leaf), masks (16 words/leaf, bit v = voxel v active).
+ * Leaves are in allocation order.  Synchronous. */
+int nm_map_download(nm_map* map, int32_t* origins, float* values,
+                    uint32_t* masks);
+/* Device pointers of the map arrays (for kernels and zero-copy readers). */
+int nm_map_device_arrays(nm_map* map, void** values, void** masks,
+                         void** origins, void** hash_keys, void** hash_vals,
+                         int64_t* hash_capacity);
+
+/* ---- plain frame path --------------------------------------------------
+ * nm_cast: filterCloud + frustumCastCloud (filterCloud.cu:476,
+ * frustumCastCloud.cu:311).  `cloud` holds n_points xyz-f32 points with
+ * byte stride `step` (PointCloud::updateData, include/nanomap/gpu/
+ * PointCloud.h:121-134).  cloud_on_device = 0 means a host pointer (copied
+ * with one H2D inside the call).  Produces per-voxel (miss, hit) counts in
+ * the context's leaf tiles. */
+int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
+            int n_points, int step, int cloud_on_device, void* stream);
+/* Handler::voxelUpdateFromFrustumBuffer / integrateTempGrid
+ * (Handler.cpp:134-182, 88-122) on the device: folds the counts of the
+ * last nm_cast into `map` and clears the tiles. */
+int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream);
+/* Manager::insertPointCloud's device work (Handler.cpp:237-284):
+ * nm_cast then nm_update_map. */
+int nm_frame_submit(nm_ctx* ctx, nm_map* map, const nm_frame* frame,
+                    const void* cloud, int n_points, int step,
+                    int cloud_on_device, void* stream);
+
+/* Stats of the last frame.  Synchronizes the context's stream. */
+int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out);
+
+/* Counts of the last nm_cast (call before nm_update_map).  Synchronous.
+ * *n_leaves receives the leaf count; arrays sized max_leaves: origins
+ * (3 ints/leaf) and miss/hit (512 per leaf), leaves in tile order. */
+int nm_counts_download(nm_ctx* ctx, int64_t max_leaves, int64_t* n_leaves,
+                       int32_t* origins, uint32_t* miss, uint32_t* hit);
+
+/* Rays produced by the voxel filter of the last nm_cast with
+ * filter = NM_FILTER_VOXEL: (dir xyz, length) per ray, index units.
+ * Synchronous. */
+int nm_filter_rays_download(nm_ctx* ctx, int max_rays, int* n_rays,
+                            float* rays4);
+
+/* ---- stage timing -------------------------------------------------------
+ * When enabled, nm_cast/nm_update_map record a HIP event pair around each
+ * stage on the call's stream.  nm_ctx_stage_times synchronizes and returns,
+ * per stage, the summed milliseconds and the number of timed launches since
+ * the last reset.  Stage ids: NM_STAGE_*. */
+#define NM_STAGE_FILTER 0     /* voxel filter (bin sort + centroids)       */
+#define NM_STAGE_DISCOVER 1   /* ray set-up + DDA, leaf discovery          */
+#define NM_STAGE_COMPACT 2    /* leaf table prefix sum                     */
+#define NM_STAGE_HITSORT 3    /* hit keys, radix sort, hit segments        */
+#define NM_STAGE_ACCUMULATE 4 /* DDA again, miss counts into leaf tiles    */
+#define NM_STAGE_UPDATE 5     /* temp values, clamped log-odds map update  */
+#define NM_STAGE_COUNT 6
+int nm_ctx_set_timing(nm_ctx* ctx, int enable);
+int nm_ctx_stage_times(nm_ctx* ctx, float* ms, int64_t* launches);
+int nm_ctx_reset_timing(nm_ctx* ctx);
+
+/* ---- host-side helpers (no device work) ------------------------------- */
+/* fold(v, a, k): v <- round(v + a), k times, in `rounding` (FP32). */
+float nm_fold_host(float v, float a, uint32_t k, int rounding);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NANOMAP_AMD_H */

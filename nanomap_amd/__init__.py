@@ -1,0 +1,276 @@
+"""nanomap_amd -- MI355X-native rebuild of NanoMap's frustum-sensor path.
+
+Python front end over the C ABI (include/nanomap_amd.h): a per-sensor
+device Context, a device-resident Map, and seeded synthetic scenes.  The
+C++ mirror of the reference API (Manager, Map, SensorData, Config, Pose)
+lives in nanomap_amd/host; this module exposes the same pipeline to
+Python for tests, benchmarks and the differentiable soft march.
+"""
+import ctypes
+
+import numpy as np
+
+from ._capi import (FILTER_NONE, FILTER_VOXEL, ROUND_DOWN, ROUND_NEAREST,
+                    Camera, CtxConfig, Frame, FrameStats, NanoMapError,
+                    _ptr, check, lib)
+
+__all__ = ["Context", "Map", "Scene", "Camera", "Frame", "FrameStats",
+           "frame_from_pose", "fold", "NanoMapError", "ROUND_DOWN",
+           "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "DEFAULT_F"]
+
+# FrameTransform of the reference's sample sensor (config/sensors/
+# 640x480.txt:22-25, column i = file line i): optical z -> body x.
+DEFAULT_F = np.array([[0.0, 0.0, 1.0],
+                      [-1.0, 0.0, 0.0],
+                      [0.0, 1.0, 0.0]], dtype=np.float32)
+
+
+def fold(v, a, k, rounding):
+    """round(v + a) applied k times in FP32 (host build of nm::fold)."""
+    return lib().nm_fold_host(float(v), float(a), int(k), int(rounding))
+
+
+def frame_from_pose(pos, quat, F=DEFAULT_F, grid_res=0.1, max_range=5.0,
+                    logodds_hit=0.85, logodds_miss=-0.4, rounding=ROUND_DOWN,
+                    width=0, height=0, filter=FILTER_NONE):
+    """Per-frame host set-up (R, eye, max_time) as the reference does it."""
+    f = Frame()
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    q = np.ascontiguousarray(quat, dtype=np.float32)
+    Fm = np.ascontiguousarray(F, dtype=np.float32).reshape(9)
+    check(lib().nm_frame_from_pose(_ptr(p, ctypes.c_float),
+                                   _ptr(q, ctypes.c_float),
+                                   _ptr(Fm, ctypes.c_float), grid_res,
+                                   max_range, logodds_hit, logodds_miss,
+                                   rounding, ctypes.byref(f)))
+    f.width, f.height, f.filter = width, height, filter
+    return f
+
+
+class Context:
+    """Per-sensor device workspace (SensorAllocator + SensorBucket)."""
+
+    def __init__(self, max_points, max_time, device=0, tile_capacity=0):
+        cfg = CtxConfig(device, int(max_points), float(max_time),
+                        int(tile_capacity))
+        h = ctypes.c_void_p()
+        check(lib().nm_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().nm_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _cloud_args(cloud, step):
+        """(pointer, n_points, step, on_device) for a numpy array, a torch
+        tensor or a (device_ptr, n_points, step) tuple."""
+        if isinstance(cloud, tuple):
+            ptr, n, st = cloud
+            return ctypes.c_void_p(int(ptr)), int(n), int(st), 1
+        if hasattr(cloud, "data_ptr"):  # torch tensor
+            n = cloud.shape[0]
+            st = step or cloud.stride(0) * cloud.element_size()
+            return ctypes.c_void_p(cloud.data_ptr()), n, st, int(cloud.is_cuda)
+        a = np.ascontiguousarray(cloud, dtype=np.float32)
+        a = a.reshape(a.shape[0], -1)
+        return a.ctypes.data_as(ctypes.c_void_p), a.shape[0], a.shape[1] * 4, 0
+
+    def cast(self, frame, cloud, step=0, stream=None):
+        ptr, n, st, dev = self._cloud_args(cloud, step)
+        self._keep = cloud
+        check(lib().nm_cast(self.handle, ctypes.byref(frame), ptr, n, st, dev,
+                            stream))
+
+    def update(self, m, stream=None):
+        check(lib().nm_update_map(self.handle, m.handle, stream))
+
+    def submit(self, m, frame, cloud, step=0, stream=None):
+        ptr, n, st, dev = self._cloud_args(cloud, step)
+        self._keep = cloud
+        check(lib().nm_frame_submit(self.handle, m.handle, ctypes.byref(frame),
+                                    ptr, n, st, dev, stream))
+
+    def stats(self):
+        s = FrameStats()
+        check(lib().nm_frame_stats_get(self.handle, ctypes.byref(s)))
+        return s.as_dict()
+
+    def counts(self, max_leaves=1 << 20):
+        """Per-voxel (miss, hit) counts of the last cast, as sorted arrays
+        (ijk [n,3], miss [n], hit [n]) over touched voxels."""
+        n = ctypes.c_int64()
+        st = lib().nm_counts_download(self.handle, 0, ctypes.byref(n), None,
+                                      None, None)
+        if st not in (0, 3):
+            check(st)
+        L = n.value
+        origins = np.zeros((max(L, 1), 3), np.int32)
+        miss = np.zeros((max(L, 1), 512), np.uint32)
+        hit = np.zeros((max(L, 1), 512), np.uint32)
+        check(lib().nm_counts_download(self.handle, L, ctypes.byref(n),
+                                       _ptr(origins, ctypes.c_int32),
+                                       _ptr(miss, ctypes.c_uint32),
+                                       _ptr(hit, ctypes.c_uint32)))
+        return _leaves_to_voxels(origins[:L], miss[:L], hit[:L])
+
+    STAGES = ("filter", "discover", "compact", "hitsort", "accumulate",
+              "update")
+
+    def set_timing(self, enable):
+        check(lib().nm_ctx_set_timing(self.handle, int(bool(enable))))
+
+    def reset_timing(self):
+        check(lib().nm_ctx_reset_timing(self.handle))
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} since the last reset."""
+        ms = np.zeros(6, np.float32)
+        n = np.zeros(6, np.int64)
+        check(lib().nm_ctx_stage_times(self.handle, _ptr(ms, ctypes.c_float),
+                                       _ptr(n, ctypes.c_int64)))
+        return {s: (float(ms[i]), int(n[i])) for i, s in enumerate(self.STAGES)}
+
+    def filter_rays(self):
+        n = ctypes.c_int()
+        st = lib().nm_filter_rays_download(self.handle, 0, ctypes.byref(n),
+                                           None)
+        if st not in (0, 3):
+            check(st)
+        out = np.zeros((max(n.value, 1), 4), np.float32)
+        check(lib().nm_filter_rays_download(self.handle, n.value,
+                                            ctypes.byref(n),
+                                            _ptr(out, ctypes.c_float)))
+        return out[:n.value]
+
+
+def _local_offsets():
+    li = np.arange(512)
+    return np.stack([li >> 6, (li >> 3) & 7, li & 7], axis=1).astype(np.int32)
+
+
+def _leaves_to_voxels(origins, miss, hit):
+    """Flatten leaf tiles to touched voxels sorted by (i, j, k)."""
+    if origins.shape[0] == 0:
+        return (np.zeros((0, 3), np.int32), np.zeros(0, np.uint32),
+                np.zeros(0, np.uint32))
+    ijk = (origins[:, None, :] + _local_offsets()[None, :, :]).reshape(-1, 3)
+    m = miss.reshape(-1)
+    h = hit.reshape(-1)
+    sel = (m | h) != 0
+    ijk, m, h = ijk[sel], m[sel], h[sel]
+    order = np.lexsort((ijk[:, 2], ijk[:, 1], ijk[:, 0]))
+    return ijk[order], m[order], h[order]
+
+
+class Map:
+    """Device-resident sparse log-odds map (map::Map)."""
+
+    def __init__(self, grid_res, clamp_min=-2.0, clamp_max=3.5, hit_thr=0.0,
+                 miss_thr=0.0, leaf_capacity=1 << 18, device=0):
+        h = ctypes.c_void_p()
+        check(lib().nm_map_create(device, grid_res, clamp_min, clamp_max,
+                                  hit_thr, miss_thr, int(leaf_capacity),
+                                  ctypes.byref(h)))
+        self.handle = h
+        self.grid_res = grid_res
+
+    def close(self):
+        if self.handle:
+            lib().nm_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self, stream=None):
+        check(lib().nm_map_clear(self.handle, stream))
+
+    def leaf_count(self):
+        n = ctypes.c_int64()
+        check(lib().nm_map_leaf_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def download(self):
+        """(origins [L,3], values [L,512], masks [L,16]) sorted by origin."""
+        L = self.leaf_count()
+        o = np.zeros((max(L, 1), 3), np.int32)
+        v = np.zeros((max(L, 1), 512), np.float32)
+        m = np.zeros((max(L, 1), 16), np.uint32)
+        check(lib().nm_map_download(self.handle, _ptr(o, ctypes.c_int32),
+                                    _ptr(v, ctypes.c_float),
+                                    _ptr(m, ctypes.c_uint32)))
+        o, v, m = o[:L], v[:L], m[:L]
+        order = np.lexsort((o[:, 2], o[:, 1], o[:, 0]))
+        return o[order], v[order], m[order]
+
+
+class Scene:
+    """Seeded box + cylinder room (nm_scene_*)."""
+
+    def __init__(self, seed, world, n_boxes, box_min=0.3, box_max=2.0,
+                 n_cylinders=0, cyl_rmin=0.1, cyl_rmax=0.5):
+        h = ctypes.c_void_p()
+        w = (ctypes.c_double * 3)(*world)
+        check(lib().nm_scene_create(seed, w, n_boxes, box_min, box_max,
+                                    n_cylinders, cyl_rmin, cyl_rmax,
+                                    ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().nm_scene_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def render(self, cam, pos, quat, seed):
+        out = np.zeros((cam.height * cam.width, 3), np.float32)
+        p = np.ascontiguousarray(pos, dtype=np.float32)
+        q = np.ascontiguousarray(quat, dtype=np.float32)
+        check(lib().nm_scene_render(self.handle, ctypes.byref(cam),
+                                    _ptr(p, ctypes.c_float),
+                                    _ptr(q, ctypes.c_float), seed,
+                                    _ptr(out, ctypes.c_float)))
+        return out
+
+    def sample_pose(self, seed, max_pitch_deg=15.0, clearance=0.3):
+        p = np.zeros(3, np.float32)
+        q = np.zeros(4, np.float32)
+        check(lib().nm_scene_sample_pose(self.handle, seed, max_pitch_deg,
+                                         clearance, _ptr(p, ctypes.c_float),
+                                         _ptr(q, ctypes.c_float)))
+        return p, q
+
+    def walk(self, seed, n, step_m=0.1, step_deg=2.0, max_pitch_deg=15.0,
+             clearance=0.3):
+        p = np.zeros((n, 3), np.float32)
+        q = np.zeros((n, 4), np.float32)
+        check(lib().nm_scene_walk(self.handle, seed, n, step_m, step_deg,
+                                  max_pitch_deg, clearance,
+                                  _ptr(p, ctypes.c_float),
+                                  _ptr(q, ctypes.c_float)))
+        return p, q
+
+
+def camera(width, height, hfov_deg, vfov_deg, max_range, noise_k=0.002,
+           dropout=0.0, F=DEFAULT_F):
+    c = Camera()
+    c.width, c.height = width, height
+    c.hfov_deg, c.vfov_deg = hfov_deg, vfov_deg
+    c.max_range, c.noise_k, c.dropout = max_range, noise_k, dropout
+    for i, x in enumerate(np.asarray(F, np.float32).reshape(9)):
+        c.frame[i] = float(x)
+    return c

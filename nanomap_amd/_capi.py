@@ -1,0 +1,149 @@
+"""ctypes binding of the C ABI in include/nanomap_amd.h and
+include/nanomap_amd_scene.h.
+
+The shared library is built in-tree (nanomap_amd/libnanomap_amd.so).  If it
+is missing, every entry point raises: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+NM_OK = 0
+NM_EINVAL = 1
+NM_ENOMEM = 2
+NM_EOVERFLOW = 3
+NM_EHIP = 4
+NM_ENODEV = 5
+NM_ESTATE = 6
+
+ROUND_NEAREST = 0
+ROUND_DOWN = 1
+FILTER_NONE = 0
+FILTER_VOXEL = 1
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "libnanomap_amd.so")
+
+
+class NanoMapError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nanomap_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class CtxConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("max_points", ctypes.c_int),
+                ("max_time", ctypes.c_float), ("tile_capacity", ctypes.c_int)]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("rotation", ctypes.c_float * 9), ("eye", ctypes.c_double * 3),
+                ("grid_res", ctypes.c_float), ("max_time", ctypes.c_float),
+                ("logodds_hit", ctypes.c_float),
+                ("logodds_miss", ctypes.c_float), ("rounding", ctypes.c_int),
+                ("filter", ctypes.c_int), ("width", ctypes.c_int),
+                ("height", ctypes.c_int)]
+
+    def copy(self):
+        f = Frame()
+        ctypes.memmove(ctypes.byref(f), ctypes.byref(self), ctypes.sizeof(Frame))
+        return f
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_int64), ("rays", ctypes.c_int64),
+                ("visits", ctypes.c_int64), ("voxels", ctypes.c_int64),
+                ("leaves", ctypes.c_int64), ("hit_voxels", ctypes.c_int64),
+                ("mixed_voxels", ctypes.c_int64),
+                ("map_leaves", ctypes.c_int64), ("errors", ctypes.c_int32),
+                ("overflow", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
+                ("max_range", ctypes.c_float), ("noise_k", ctypes.c_float),
+                ("dropout", ctypes.c_float), ("frame", ctypes.c_float * 9)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_D = ctypes.c_double
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PU32 = ctypes.POINTER(ctypes.c_uint32)
+_PF = ctypes.POINTER(ctypes.c_float)
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); every symbol of the public headers.
+SIGNATURES = {
+    "nm_last_error": (ctypes.c_char_p, []),
+    "nm_version": (_I, []),
+    "nm_ctx_create": (_I, [ctypes.POINTER(CtxConfig), ctypes.POINTER(_P)]),
+    "nm_ctx_destroy": (_I, [_P]),
+    "nm_map_create": (_I, [_I, _F, _F, _F, _F, _F, _I, ctypes.POINTER(_P)]),
+    "nm_map_destroy": (_I, [_P]),
+    "nm_map_clear": (_I, [_P, _P]),
+    "nm_map_leaf_count": (_I, [_P, _PI64]),
+    "nm_map_download": (_I, [_P, _PI32, _PF, _PU32]),
+    "nm_map_device_arrays": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                  ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                  ctypes.POINTER(_P), _PI64]),
+    "nm_cast": (_I, [_P, ctypes.POINTER(Frame), _P, _I, _I, _I, _P]),
+    "nm_update_map": (_I, [_P, _P, _P]),
+    "nm_frame_submit": (_I, [_P, _P, ctypes.POINTER(Frame), _P, _I, _I, _I,
+                             _P]),
+    "nm_frame_stats_get": (_I, [_P, ctypes.POINTER(FrameStats)]),
+    "nm_counts_download": (_I, [_P, ctypes.c_int64, _PI64, _PI32, _PU32,
+                                _PU32]),
+    "nm_filter_rays_download": (_I, [_P, _I, ctypes.POINTER(_I), _PF]),
+    "nm_fold_host": (_F, [_F, _F, ctypes.c_uint32, _I]),
+    "nm_ctx_set_timing": (_I, [_P, _I]),
+    "nm_ctx_stage_times": (_I, [_P, _PF, _PI64]),
+    "nm_ctx_reset_timing": (_I, [_P]),
+    "nm_scene_create": (_I, [ctypes.c_uint64, _PD, _I, _D, _D, _I, _D, _D,
+                             ctypes.POINTER(_P)]),
+    "nm_scene_destroy": (_I, [_P]),
+    "nm_scene_render": (_I, [_P, ctypes.POINTER(Camera), _PF, _PF,
+                             ctypes.c_uint64, _PF]),
+    "nm_scene_sample_pose": (_I, [_P, ctypes.c_uint64, _F, _F, _PF, _PF]),
+    "nm_scene_walk": (_I, [_P, ctypes.c_uint64, _I, _F, _F, _F, _F, _PF,
+                           _PF]),
+    "nm_frame_from_pose": (_I, [_PF, _PF, _PF, _F, _F, _F, _F, _I,
+                                ctypes.POINTER(Frame)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libnanomap_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NanoMapError(NM_ENODEV,
+                               "%s is not built: run __graft_entry__.build()"
+                               % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != NM_OK:
+        msg = lib().nm_last_error()
+        raise NanoMapError(status, msg.decode() if msg else "")
+    return status
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,608 @@
+// nm_capi.hip -- the extern "C" entry points of include/nanomap_amd.h.
+//
+// Host orchestration of one frame, replacing Handler::processPointCloud's
+// GPU branch (src/nanomap/handler/Handler.cpp:250-278): the reference
+// synchronizes the device three times per frame to learn the ray and leaf
+// counts (filterCloud.cu:550-551, frustumCastCloud.cu:347-356) and
+// allocates CUB scratch every frame (frustumCastCloud.cu:64-68).  Here
+// every count stays on the device, kernels are sized by worst-case
+// capacities allocated once per context, and a frame is a fixed sequence
+// of launches on one stream with no host synchronization.
+#include <math.h>
+#include <algorithm>
+#include <cmath>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "nm_fp.h"
+#include "nm_internal.h"
+
+struct nm_ctx {
+  int device = 0;
+  int max_points = 0;
+  float max_time = 0;
+  int cube_side_max = 0;
+  int64_t cube_cap = 0;
+  int tile_cap = 0;
+  size_t staging_bytes = 0;
+  nm::CtxBuffers b{};
+  nm::FrameCounters* host_counters = nullptr;
+  // state of the last nm_cast
+  bool pending = false;
+  nm_frame frame{};
+  nm::Cube cube{};
+  int n = 0;
+  int sorted_buf = 0;
+  hipStream_t stream = nullptr;
+  nm_map* last_map = nullptr;
+  // stage timing
+  bool timing = false;
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;          // recorded, not yet read
+  std::vector<hipEvent_t> free_ev;  // event pool
+  double stage_ms[NM_STAGE_COUNT] = {};
+  int64_t stage_n[NM_STAGE_COUNT] = {};
+};
+
+struct nm_map {
+  int device = 0;
+  float grid_res = 0;
+  nm::MapView v{};
+  hipStream_t stream = nullptr;
+};
+
+namespace nm {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+}  // namespace nm
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  nm::set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(NM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NM_HIP(call)                                 \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, count * sizeof(T) + 64);
+}
+
+int sort_bits_for(int64_t max_key) {
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) <= max_key) ++bits;
+  return bits;
+}
+
+nm::Cube frame_cube(const nm_frame& f) {
+  nm::Cube c;
+  const int r = (int)ceil(((double)f.max_time + 2.0) / 8.0) + 1;
+  c.side = 2 * r + 1;
+  for (int a = 0; a < 3; ++a) {
+    const int v = (int)floor(f.eye[a]);
+    c.cmin[a] = (v >> 3) - r;
+  }
+  return c;
+}
+
+hipEvent_t take_event(nm_ctx* c) {
+  if (!c->free_ev.empty()) {
+    hipEvent_t e = c->free_ev.back();
+    c->free_ev.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// RAII event pair around one stage (no-op unless timing is enabled).
+struct StageTimer {
+  nm_ctx* c;
+  int stage;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  StageTimer(nm_ctx* c_, int stage_, hipStream_t s_) : c(c_), stage(stage_), s(s_) {
+    if (!c->timing) return;
+    a = take_event(c);
+    if (a) (void)hipEventRecord(a, s);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    c->marks.push_back({stage, a, b});
+  }
+};
+
+int drain_marks(nm_ctx* c) {
+  for (auto& m : c->marks) {
+    float ms = 0;
+    hipError_t e = hipEventSynchronize(m.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, m.a, m.b);
+    if (e != hipSuccess) return hip_fail(e, "stage timing");
+    c->stage_ms[m.stage] += ms;
+    c->stage_n[m.stage] += 1;
+    c->free_ev.push_back(m.a);
+    c->free_ev.push_back(m.b);
+  }
+  c->marks.clear();
+  return NM_OK;
+}
+
+void free_ctx(nm_ctx* c) {
+  for (auto& m : c->marks) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
+  nm::CtxBuffers& b = c->b;
+  void* ptrs[] = {b.leaf_slot,  b.slot_leaf,  b.tiles,      b.hit_mask,
+                  b.ray_hit,    b.keys[0],    b.keys[1],    b.vals[0],
+                  b.vals[1],    b.gap,        b.radix_hist, b.scan_blocks,
+                  b.counters,   b.cloud,      b.filter_rays, b.filter_tmp,
+                  b.flags};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->host_counters) (void)hipHostFree(c->host_counters);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nm_last_error(void) { return nm::g_error.c_str(); }
+
+int nm_version(void) { return 1; }
+
+float nm_fold_host(float v, float a, uint32_t k, int rounding) {
+  return rounding == NM_ROUND_DOWN ? nm::fold<nm::kDown>(v, a, k)
+                                   : nm::fold<nm::kNearest>(v, a, k);
+}
+
+int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
+  if (!cfg || !out) return fail(NM_EINVAL, "nm_ctx_create: null argument");
+  *out = nullptr;
+  if (cfg->max_points <= 0 || !(cfg->max_time > 0) || cfg->max_time > 4000)
+    return fail(NM_EINVAL, "nm_ctx_create: bad max_points or max_time");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(NM_ENODEV, "nm_ctx_create: no HIP device");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(NM_EINVAL, "nm_ctx_create: bad device ordinal");
+  NM_HIP(hipSetDevice(cfg->device));
+  nm_ctx* c = new nm_ctx();
+  c->device = cfg->device;
+  c->max_points = cfg->max_points;
+  c->max_time = cfg->max_time;
+  const int r = (int)ceil(((double)cfg->max_time + 2.0) / 8.0) + 1;
+  c->cube_side_max = 2 * r + 1;
+  c->cube_cap = (int64_t)c->cube_side_max * c->cube_side_max * c->cube_side_max;
+  if (c->cube_cap >= (1 << 22)) {
+    delete c;
+    return fail(NM_EINVAL, "nm_ctx_create: max_time too large for the leaf cube");
+  }
+  c->tile_cap = cfg->tile_capacity > 0 ? cfg->tile_capacity : (int)c->cube_cap;
+  if ((int64_t)c->tile_cap > c->cube_cap) c->tile_cap = (int)c->cube_cap;
+  nm::CtxBuffers& b = c->b;
+  const size_t P = (size_t)cfg->max_points;
+  const size_t scan_blocks =
+      (size_t)std::max<int64_t>(c->cube_cap, (int64_t)P) / nm::kScanChunk + 2;
+  c->staging_bytes = P * 16;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  A(dalloc(&b.leaf_slot, (size_t)c->cube_cap));
+  A(dalloc(&b.slot_leaf, (size_t)c->tile_cap));
+  A(dalloc(&b.tiles, (size_t)c->tile_cap * 1024));
+  A(dalloc(&b.hit_mask, (size_t)c->tile_cap * nm::kMaskWords));
+  A(dalloc(&b.ray_hit, P));
+  A(dalloc(&b.keys[0], P));
+  A(dalloc(&b.keys[1], P));
+  A(dalloc(&b.vals[0], P));
+  A(dalloc(&b.vals[1], P));
+  A(dalloc(&b.gap, P));
+  A(dalloc(&b.radix_hist, (size_t)256 * (nm::radix_max_blocks((int)P) + 1)));
+  A(dalloc(&b.scan_blocks, scan_blocks));
+  A(dalloc(&b.counters, 1));
+  A(dalloc(&b.cloud, c->staging_bytes));
+  A(dalloc(&b.filter_rays, P * 4));
+  A(dalloc(&b.filter_tmp, P * 4));
+  A(dalloc(&b.flags, P));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&c->host_counters, sizeof(nm::FrameCounters));
+  if (e == hipSuccess)
+    e = hipMemset(b.leaf_slot, 0xff, (size_t)c->cube_cap * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMemset(b.tiles, 0, (size_t)c->tile_cap * 1024 * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemset(b.hit_mask, 0,
+                  (size_t)c->tile_cap * nm::kMaskWords * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(b.counters, 0, sizeof(nm::FrameCounters));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    free_ctx(c);
+    return fail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
+                std::string("nm_ctx_create: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return NM_OK;
+}
+
+int nm_ctx_destroy(nm_ctx* ctx) {
+  if (!ctx) return NM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  free_ctx(ctx);
+  return NM_OK;
+}
+
+int nm_map_create(int device, float grid_res, float clamp_min,
+                  float clamp_max, float hit_thr, float miss_thr,
+                  int leaf_capacity, nm_map** out) {
+  if (!out) return fail(NM_EINVAL, "nm_map_create: null out");
+  *out = nullptr;
+  if (!(grid_res > 0) || leaf_capacity <= 0 || !(clamp_min <= clamp_max))
+    return fail(NM_EINVAL, "nm_map_create: bad argument");
+  NM_HIP(hipSetDevice(device));
+  nm_map* m = new nm_map();
+  m->device = device;
+  m->grid_res = grid_res;
+  nm::MapView& v = m->v;
+  v.leaf_cap = leaf_capacity;
+  int64_t h = 1;
+  while (h < 2 * (int64_t)leaf_capacity) h <<= 1;
+  v.hash_cap = h;
+  v.clamp_min = clamp_min;
+  v.clamp_max = clamp_max;
+  v.hit_thr = hit_thr;
+  v.miss_thr = miss_thr;
+  hipError_t e = hipSuccess;
+  auto A = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  A(dalloc(&v.hash_keys, (size_t)h));
+  A(dalloc(&v.hash_vals, (size_t)h));
+  A(dalloc(&v.values, (size_t)leaf_capacity * 512));
+  A(dalloc(&v.masks, (size_t)leaf_capacity * nm::kMaskWords));
+  A(dalloc(&v.origins, (size_t)leaf_capacity * 3));
+  A(dalloc(&v.leaf_count, 1));
+  if (e == hipSuccess) e = nm::launch_map_clear(v, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    nm_map_destroy(m);
+    return fail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
+                std::string("nm_map_create: ") + hipGetErrorString(e));
+  }
+  *out = m;
+  return NM_OK;
+}
+
+int nm_map_destroy(nm_map* m) {
+  if (!m) return NM_OK;
+  (void)hipSetDevice(m->device);
+  (void)hipDeviceSynchronize();
+  void* ptrs[] = {m->v.hash_keys, m->v.hash_vals, m->v.values,
+                  m->v.masks,     m->v.origins,   m->v.leaf_count};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+  return NM_OK;
+}
+
+int nm_map_clear(nm_map* m, void* stream) {
+  if (!m) return fail(NM_EINVAL, "nm_map_clear: null map");
+  NM_HIP(hipSetDevice(m->device));
+  NM_HIP(nm::launch_map_clear(m->v, (hipStream_t)stream));
+  m->stream = (hipStream_t)stream;
+  return NM_OK;
+}
+
+int nm_map_leaf_count(nm_map* m, int64_t* count) {
+  if (!m || !count) return fail(NM_EINVAL, "nm_map_leaf_count: null argument");
+  NM_HIP(hipSetDevice(m->device));
+  NM_HIP(hipStreamSynchronize(m->stream));
+  int32_t n = 0;
+  NM_HIP(hipMemcpy(&n, m->v.leaf_count, sizeof(n), hipMemcpyDeviceToHost));
+  *count = std::min<int64_t>(n, m->v.leaf_cap);
+  return NM_OK;
+}
+
+int nm_map_download(nm_map* m, int32_t* origins, float* values,
+                    uint32_t* masks) {
+  int64_t n = 0;
+  int st = nm_map_leaf_count(m, &n);
+  if (st != NM_OK) return st;
+  if (n == 0) return NM_OK;
+  if (origins)
+    NM_HIP(hipMemcpy(origins, m->v.origins, (size_t)n * 3 * sizeof(int32_t),
+                     hipMemcpyDeviceToHost));
+  if (values)
+    NM_HIP(hipMemcpy(values, m->v.values, (size_t)n * 512 * sizeof(float),
+                     hipMemcpyDeviceToHost));
+  if (masks)
+    NM_HIP(hipMemcpy(masks, m->v.masks,
+                     (size_t)n * nm::kMaskWords * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost));
+  return NM_OK;
+}
+
+int nm_map_device_arrays(nm_map* m, void** values, void** masks,
+                         void** origins, void** hash_keys, void** hash_vals,
+                         int64_t* hash_capacity) {
+  if (!m) return fail(NM_EINVAL, "nm_map_device_arrays: null map");
+  if (values) *values = m->v.values;
+  if (masks) *masks = m->v.masks;
+  if (origins) *origins = m->v.origins;
+  if (hash_keys) *hash_keys = m->v.hash_keys;
+  if (hash_vals) *hash_vals = m->v.hash_vals;
+  if (hash_capacity) *hash_capacity = m->v.hash_cap;
+  return NM_OK;
+}
+
+int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
+            int n_points, int step, int cloud_on_device, void* stream) {
+  if (!ctx || !frame) return fail(NM_EINVAL, "nm_cast: null argument");
+  if (n_points < 0 || n_points > ctx->max_points)
+    return fail(NM_EINVAL, "nm_cast: n_points exceeds the context capacity");
+  if (n_points > 0 && (!cloud || step < 12 || (step & 3)))
+    return fail(NM_EINVAL, "nm_cast: cloud must be xyz f32 with a stride >= 12, multiple of 4");
+  if (!(frame->grid_res > 0) || !(frame->max_time > 0) ||
+      frame->max_time > ctx->max_time)
+    return fail(NM_EINVAL, "nm_cast: max_time must be in (0, context max_time]");
+  if (frame->rounding != NM_ROUND_NEAREST && frame->rounding != NM_ROUND_DOWN)
+    return fail(NM_EINVAL, "nm_cast: bad rounding mode");
+  if (frame->filter != NM_FILTER_NONE && frame->filter != NM_FILTER_VOXEL)
+    return fail(NM_EINVAL, "nm_cast: bad filter type");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(frame->eye[a]) || fabs(frame->eye[a]) > 1e8)
+      return fail(NM_EINVAL, "nm_cast: eye must be finite");
+  if (ctx->pending)
+    return fail(NM_ESTATE, "nm_cast: the previous cast was not folded into a map (nm_update_map)");
+  NM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  const nm::Cube cube = frame_cube(*frame);
+  if (cube.side > ctx->cube_side_max)
+    return fail(NM_EINVAL, "nm_cast: frame cube exceeds the context");
+
+  const unsigned char* pts = (const unsigned char*)cloud;
+  if (!cloud_on_device && n_points > 0) {
+    const size_t bytes = (size_t)n_points * (size_t)step;
+    if (bytes > ctx->staging_bytes) {
+      NM_HIP(hipStreamSynchronize(s));
+      NM_HIP(hipFree(ctx->b.cloud));
+      ctx->b.cloud = nullptr;
+      NM_HIP(hipMalloc((void**)&ctx->b.cloud, bytes));
+      ctx->staging_bytes = bytes;
+    }
+    NM_HIP(hipMemcpyAsync(ctx->b.cloud, cloud, bytes, hipMemcpyHostToDevice, s));
+    pts = ctx->b.cloud;
+  }
+
+  nm::CastParams p{};
+  memcpy(p.R, frame->rotation, sizeof(p.R));
+  memcpy(p.eye, frame->eye, sizeof(p.eye));
+  p.res = frame->grid_res;
+  p.max_time = frame->max_time;
+  p.hit = frame->logodds_hit;
+  p.miss = frame->logodds_miss;
+  p.cube = cube;
+  p.cloud = pts;
+  p.n = n_points;
+  p.step = step;
+  p.width = frame->width;
+  p.use_rays = 0;
+  p.rays4 = nullptr;
+  p.n_dev = nullptr;
+
+  NM_HIP(nm::launch_reset_counters(ctx->b.counters, s));
+  if (frame->filter == NM_FILTER_VOXEL) {
+    StageTimer t(ctx, NM_STAGE_FILTER, s);
+    float origin[3];
+    for (int a = 0; a < 3; ++a)
+      origin[a] = (float)(frame->eye[a] - floor(frame->eye[a]));
+    NM_HIP(nm::launch_voxel_filter(p, origin, ctx->b, s));
+    p.use_rays = 1;
+    p.rays4 = ctx->b.filter_rays;
+    p.n_dev = &ctx->b.counters->n_filter_rays;
+  }
+  const int64_t vol = (int64_t)cube.side * cube.side * cube.side;
+  {
+    StageTimer t(ctx, NM_STAGE_DISCOVER, s);
+    NM_HIP(nm::launch_discover(frame->rounding, p, ctx->b, s));
+  }
+  {
+    StageTimer t(ctx, NM_STAGE_COMPACT, s);
+    NM_HIP(nm::launch_compact_leaves(ctx->b, vol, ctx->tile_cap, s));
+  }
+  int sorted = 0;
+  {
+    StageTimer t(ctx, NM_STAGE_HITSORT, s);
+    NM_HIP(nm::launch_hit_keys(p, ctx->b, s));
+    const int bits = sort_bits_for((int64_t)ctx->tile_cap * 512);
+    NM_HIP(nm::radix_sort_pairs(ctx->b.keys, ctx->b.vals, n_points, bits,
+                                ctx->b.radix_hist, &sorted, s));
+    NM_HIP(nm::launch_hit_segments(ctx->b, n_points, sorted, s));
+  }
+  {
+    StageTimer t(ctx, NM_STAGE_ACCUMULATE, s);
+    NM_HIP(nm::launch_accumulate(frame->rounding, p, ctx->b, sorted, s));
+  }
+  ctx->pending = true;
+  ctx->frame = *frame;
+  ctx->cube = cube;
+  ctx->n = n_points;
+  ctx->sorted_buf = sorted;
+  ctx->stream = s;
+  return NM_OK;
+}
+
+int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
+  if (!ctx || !map) return fail(NM_EINVAL, "nm_update_map: null argument");
+  if (!ctx->pending) return fail(NM_ESTATE, "nm_update_map: no pending cast");
+  if (map->device != ctx->device)
+    return fail(NM_EINVAL, "nm_update_map: map and context on different devices");
+  NM_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  {
+    StageTimer t(ctx, NM_STAGE_UPDATE, s);
+    NM_HIP(nm::launch_update(ctx->frame.rounding, ctx->cube, ctx->b, map->v,
+                             ctx->frame.logodds_hit, ctx->frame.logodds_miss,
+                             ctx->sorted_buf, ctx->tile_cap, s));
+  }
+  ctx->pending = false;
+  ctx->last_map = map;
+  ctx->stream = s;
+  map->stream = s;
+  return NM_OK;
+}
+
+int nm_frame_submit(nm_ctx* ctx, nm_map* map, const nm_frame* frame,
+                    const void* cloud, int n_points, int step,
+                    int cloud_on_device, void* stream) {
+  int st = nm_cast(ctx, frame, cloud, n_points, step, cloud_on_device, stream);
+  if (st != NM_OK) return st;
+  return nm_update_map(ctx, map, stream);
+}
+
+int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out) {
+  if (!ctx || !out) return fail(NM_EINVAL, "nm_frame_stats_get: null argument");
+  NM_HIP(hipSetDevice(ctx->device));
+  NM_HIP(hipMemcpyAsync(ctx->host_counters, ctx->b.counters,
+                        sizeof(nm::FrameCounters), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  NM_HIP(hipStreamSynchronize(ctx->stream));
+  const nm::FrameCounters& c = *ctx->host_counters;
+  memset(out, 0, sizeof(*out));
+  out->points = ctx->n;
+  out->rays = (int64_t)c.rays;
+  out->visits = (int64_t)c.visits;
+  out->voxels = (int64_t)c.voxels;
+  out->leaves = c.n_leaves;
+  out->hit_voxels = (int64_t)c.hit_voxels;
+  out->mixed_voxels = (int64_t)c.mixed_voxels;
+  out->errors = c.errors;
+  out->overflow = c.overflow;
+  if (ctx->last_map) {
+    int32_t n = 0;
+    NM_HIP(hipMemcpy(&n, ctx->last_map->v.leaf_count, sizeof(n),
+                     hipMemcpyDeviceToHost));
+    out->map_leaves = n;
+  }
+  return NM_OK;
+}
+
+int nm_counts_download(nm_ctx* ctx, int64_t max_leaves, int64_t* n_leaves,
+                       int32_t* origins, uint32_t* miss, uint32_t* hit) {
+  if (!ctx || !n_leaves) return fail(NM_EINVAL, "nm_counts_download: null argument");
+  if (!ctx->pending)
+    return fail(NM_ESTATE, "nm_counts_download: call between nm_cast and nm_update_map");
+  NM_HIP(hipSetDevice(ctx->device));
+  NM_HIP(hipStreamSynchronize(ctx->stream));
+  nm::FrameCounters c;
+  NM_HIP(hipMemcpy(&c, ctx->b.counters, sizeof(c), hipMemcpyDeviceToHost));
+  const int64_t L = std::min<int64_t>(c.n_leaves, ctx->tile_cap);
+  *n_leaves = L;
+  if (L > max_leaves) return fail(NM_EOVERFLOW, "nm_counts_download: arrays too small");
+  if (L == 0) return NM_OK;
+  std::string err;
+  int32_t* slot_leaf = new int32_t[L];
+  uint32_t* tiles = new uint32_t[(size_t)L * 1024];
+  hipError_t e = hipMemcpy(slot_leaf, ctx->b.slot_leaf, L * sizeof(int32_t),
+                           hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(tiles, ctx->b.tiles, (size_t)L * 1024 * sizeof(uint32_t),
+                  hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    const int side = ctx->cube.side;
+    for (int64_t s = 0; s < L; ++s) {
+      const int cidx = slot_leaf[s];
+      const int lz = cidx % side, ly = (cidx / side) % side,
+                lx = cidx / (side * side);
+      if (origins) {
+        origins[3 * s + 0] = (lx + ctx->cube.cmin[0]) * 8;
+        origins[3 * s + 1] = (ly + ctx->cube.cmin[1]) * 8;
+        origins[3 * s + 2] = (lz + ctx->cube.cmin[2]) * 8;
+      }
+      for (int v = 0; v < 512; ++v) {
+        if (miss) miss[s * 512 + v] = tiles[(s * 512 + v) * 2];
+        if (hit) hit[s * 512 + v] = tiles[(s * 512 + v) * 2 + 1];
+      }
+    }
+  }
+  delete[] slot_leaf;
+  delete[] tiles;
+  if (e != hipSuccess) return hip_fail(e, "nm_counts_download");
+  return NM_OK;
+}
+
+int nm_filter_rays_download(nm_ctx* ctx, int max_rays, int* n_rays,
+                            float* rays4) {
+  if (!ctx || !n_rays) return fail(NM_EINVAL, "nm_filter_rays_download: null argument");
+  NM_HIP(hipSetDevice(ctx->device));
+  NM_HIP(hipStreamSynchronize(ctx->stream));
+  nm::FrameCounters c;
+  NM_HIP(hipMemcpy(&c, ctx->b.counters, sizeof(c), hipMemcpyDeviceToHost));
+  *n_rays = c.n_filter_rays;
+  if (c.n_filter_rays > max_rays)
+    return fail(NM_EOVERFLOW, "nm_filter_rays_download: array too small");
+  if (c.n_filter_rays > 0 && rays4)
+    NM_HIP(hipMemcpy(rays4, ctx->b.filter_rays,
+                     (size_t)c.n_filter_rays * 4 * sizeof(float),
+                     hipMemcpyDeviceToHost));
+  return NM_OK;
+}
+
+int nm_ctx_set_timing(nm_ctx* ctx, int enable) {
+  if (!ctx) return fail(NM_EINVAL, "nm_ctx_set_timing: null context");
+  ctx->timing = enable != 0;
+  return NM_OK;
+}
+
+int nm_ctx_stage_times(nm_ctx* ctx, float* ms, int64_t* launches) {
+  if (!ctx) return fail(NM_EINVAL, "nm_ctx_stage_times: null context");
+  NM_HIP(hipSetDevice(ctx->device));
+  int st = drain_marks(ctx);
+  if (st != NM_OK) return st;
+  for (int i = 0; i < NM_STAGE_COUNT; ++i) {
+    if (ms) ms[i] = (float)ctx->stage_ms[i];
+    if (launches) launches[i] = ctx->stage_n[i];
+  }
+  return NM_OK;
+}
+
+int nm_ctx_reset_timing(nm_ctx* ctx) {
+  if (!ctx) return fail(NM_EINVAL, "nm_ctx_reset_timing: null context");
+  NM_HIP(hipSetDevice(ctx->device));
+  int st = drain_marks(ctx);
+  if (st != NM_OK) return st;
+  for (int i = 0; i < NM_STAGE_COUNT; ++i) {
+    ctx->stage_ms[i] = 0;
+    ctx->stage_n[i] = 0;
+  }
+  return NM_OK;
+}
+
+}  // extern "C"

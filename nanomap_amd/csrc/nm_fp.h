@@ -1,0 +1,255 @@
+// nm_fp.h -- IEEE arithmetic in a selectable rounding mode, computed in
+// the default round-to-nearest-even hardware mode.
+//
+// The reference executes its host path with FE_DOWNWARD left set by
+// SensorData::updatePose (include/nanomap/sensor/SensorData.h:42), so the
+// hit/miss traversal and the FP32 log-odds sums round toward -inf.  On
+// gfx950 the MODE register could select that rounding, but the compiler's
+// mode-register pass resets the FP64 rounding bits around its own div/sqrt
+// sequences without restoring them.  So round-down is emulated exactly:
+//   add: TwoSum error term; result stepped down one ulp if the error < 0;
+//   mul: FMA residual a*b - p; stepped down if < 0;
+//   div: FMA residual a - q*b of the correctly rounded quotient;
+//   sqrt: FMA residual x - s*s of the correctly rounded root.
+// Each residual is exact, so each result equals the one IEEE operation
+// rounded toward -inf.  RN instantiations are the plain operations.
+//
+// Build with -ffp-contract=off: a contracted a*b+c changes the result.
+#ifndef NM_FP_H
+#define NM_FP_H
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NM_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define NM_HD inline
+#endif
+
+namespace nm {
+
+enum Rounding : int { kNearest = 0, kDown = 1 };
+
+NM_HD uint64_t bits_of(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return u;
+}
+NM_HD double double_of(uint64_t u) {
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
+NM_HD uint32_t bits_of(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  return u;
+}
+NM_HD float float_of(uint32_t u) {
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+// Next representable value toward -inf, for finite nonzero x.
+NM_HD double next_down(double x) {
+  const uint64_t u = bits_of(x);
+  return double_of((u >> 63) ? u + 1 : u - 1);
+}
+NM_HD float next_down(float x) {
+  const uint32_t u = bits_of(x);
+  return float_of((u >> 31) ? u + 1 : u - 1);
+}
+
+NM_HD double fma_(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fma(a, b, c);
+#else
+  return std::fma(a, b, c);
+#endif
+}
+NM_HD float fma_(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fmaf(a, b, c);
+#else
+  return std::fmaf(a, b, c);
+#endif
+}
+NM_HD double sqrt_(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_sqrt(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+NM_HD float sqrt_(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_sqrtf(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+
+// Sum rounded toward -inf.  An exact zero sum of nonzero operands is -0
+// in that mode; +0 only when both operands are +0.
+template <class T>
+NM_HD T add_down(T a, T b) {
+  T s = a + b;
+  if (s == T(0)) {
+    const bool both_pos_zero = (a == T(0) && b == T(0) &&
+                                !(bits_of(a) >> (sizeof(T) * 8 - 1)) &&
+                                !(bits_of(b) >> (sizeof(T) * 8 - 1)));
+    return both_pos_zero ? T(0) : -T(0);
+  }
+  const T bb = s - a;
+  const T err = (a - (s - bb)) + (b - bb);
+  return err < T(0) ? next_down(s) : s;
+}
+
+template <class T>
+NM_HD T mul_down(T a, T b) {
+  const T p = a * b;
+  if (p == T(0)) return p;
+  const T err = fma_(a, b, -p);
+  return err < T(0) ? next_down(p) : p;
+}
+
+template <class T>
+NM_HD T div_down(T a, T b) {
+  const T q = a / b;
+  if (q == T(0) || b == T(0)) return q;
+  const T r = fma_(-q, b, a);  // exact: a - q*b
+  const bool q_above = (b > T(0)) ? (r < T(0)) : (r > T(0));
+  return q_above ? next_down(q) : q;
+}
+
+template <class T>
+NM_HD T sqrt_down(T x) {
+  const T s = sqrt_(x);
+  if (s == T(0)) return s;
+  const T r = fma_(-s, s, x);  // exact: x - s*s
+  return r < T(0) ? next_down(s) : s;
+}
+
+// Mode-dispatched operations; M is a compile-time constant in kernels.
+template <int M, class T>
+NM_HD T add(T a, T b) {
+  return M == kDown ? add_down(a, b) : a + b;
+}
+template <int M, class T>
+NM_HD T sub(T a, T b) {
+  return M == kDown ? add_down(a, -b) : a - b;
+}
+template <int M, class T>
+NM_HD T mul(T a, T b) {
+  return M == kDown ? mul_down(a, b) : a * b;
+}
+template <int M, class T>
+NM_HD T div(T a, T b) {
+  return M == kDown ? div_down(a, b) : a / b;
+}
+template <int M, class T>
+NM_HD T sqrt(T x) {
+  return M == kDown ? sqrt_down(x) : sqrt_(x);
+}
+
+// ---------------------------------------------------------------------
+// fold<M>(v, a, k): v_{i+1} = round_M(v_i + a) applied k times (FP32).
+//
+// Inside one binade [2^e, 2^(e+1)) every float is a multiple of
+// u = 2^(e-23), so while the exact sums v_i + a stay in that binade the
+// rounded step is the constant d = round_M(a / u) * u.  The loop jumps
+// over such runs in one exact multiply-add and takes single rounded steps
+// near binade boundaries, near zero and at round-to-nearest ties (where
+// the step depends on the parity of v).  Cost: O(number of binades).
+// ---------------------------------------------------------------------
+NM_HD double floor_(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_floor(x);
+#else
+  return std::floor(x);
+#endif
+}
+NM_HD double rint_(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_rint(x);
+#else
+  return std::rint(x);
+#endif
+}
+NM_HD double abs_(double x) { return x < 0 ? -x : x; }
+
+template <int M>
+NM_HD float fold(float v, float a, uint32_t k) {
+  if (a == 0.0f) return k ? add<M>(v, a) : v;
+  while (k > 0) {
+    const double av = abs_((double)v), aa = abs_((double)a);
+    // Single steps while |v| is small relative to the increment, so every
+    // jump below stays clear of zero crossings and subnormals.
+    if (av < 2.0 * aa || av < 1e-30) {
+      v = add<M>(v, a);
+      --k;
+      continue;
+    }
+    const uint32_t ex = (bits_of(v) >> 23) & 0xff;  // biased exponent >= 1
+    const double lo = double_of((uint64_t)(ex + 896) << 52);      // 2^e
+    const double ulp = double_of((uint64_t)(ex + 896 - 23) << 52);  // 2^(e-23)
+    const double hi = 2.0 * lo;
+    const double q = (double)a / ulp;  // exact: power-of-two scaling
+    double dq;
+    if (M == kDown) {
+      dq = floor_(q);
+    } else {
+      if (q - floor_(q) == 0.5) {  // tie: step depends on parity of v
+        v = add<M>(v, a);
+        --k;
+        continue;
+      }
+      dq = rint_(q);
+    }
+    const double d = dq * ulp;  // exact
+    if (d == 0.0) return v;     // every further step leaves v unchanged
+    const bool away = ((double)v > 0) == (d > 0);
+    const double ad = abs_(d);
+    int64_t n;
+    if (away) {
+      // step i valid while |v| + i|d| + |a| < 2^(e+1)
+      const double X = hi - aa - av;  // exact in double for |v| < 2^29
+      if (X <= 0) {
+        n = 0;
+      } else {
+        int64_t i = (int64_t)floor_(X / ad);
+        while (i > 0 && (double)i * ad >= X) --i;
+        while ((double)(i + 1) * ad < X) ++i;
+        n = ((double)i * ad < X) ? i + 1 : 0;
+      }
+    } else {
+      // step i valid while |v| - i|d| - |a| >= 2^e
+      const double Y = av - aa - lo;
+      if (Y < 0) {
+        n = 0;
+      } else {
+        int64_t i = (int64_t)floor_(Y / ad);
+        while (i > 0 && (double)i * ad > Y) --i;
+        while ((double)(i + 1) * ad <= Y) ++i;
+        n = i + 1;
+      }
+    }
+    if (n <= 0) {
+      v = add<M>(v, a);
+      --k;
+      continue;
+    }
+    if ((uint64_t)n > k) n = (int64_t)k;
+    v = (float)((double)v + (double)n * d);  // exact: stays on the grid
+    k -= (uint32_t)n;
+  }
+  return v;
+}
+
+}  // namespace nm
+
+#endif  // NM_FP_H

@@ -1,0 +1,44 @@
+"""Seeded synthetic workloads (nm_scene_*) for tests, benchmarks, examples.
+
+cfg1: 20x20x5 m room, 40 boxes (sides 0.3-2 m), one 160x120 frame at
+      60x45 deg, 5 m range, 0.1 m voxels, noise sigma = 0.002 d^2.
+cfg2: 50x50x10 m room, 300 boxes + 200 cylinders (r 0.1-0.5 m), a
+      100-frame random walk (0.1 m, 2 deg per frame), 640x480 at 87x58 deg,
+      10 m range, 0.05 m voxels, 5% NaN dropouts.
+Log-odds hit +0.85, miss -0.4, clamp [-2.0, 3.5], thresholds 0/0.
+"""
+import numpy as np
+
+HIT = 0.85
+MISS = -0.4
+CLAMP = (-2.0, 3.5)
+
+
+def cfg1(nm, seed=1):
+    """One 160x120 frame at 0.1 m in a 20x20x5 m box room."""
+    scene = nm.Scene(seed, (20.0, 20.0, 5.0), 40, 0.3, 2.0)
+    pos, quat = scene.sample_pose(seed, 15.0, 0.3)
+    cam = nm.camera(160, 120, 60.0, 45.0, 5.0, noise_k=0.002, dropout=0.0)
+    cloud = scene.render(cam, pos, quat, seed)
+    return scene, cam, [(pos, quat, cloud)]
+
+
+def cfg2_scene(nm, seed=2):
+    """50x50x10 m room, 640x480 camera at 87x58 deg, 10 m range."""
+    scene = nm.Scene(seed, (50.0, 50.0, 10.0), 300, 0.3, 2.0, 200, 0.1, 0.5)
+    cam = nm.camera(640, 480, 87.0, 58.0, 10.0, noise_k=0.002, dropout=0.05)
+    return scene, cam
+
+
+def cfg2_frames(nm, n_frames=100, seed=2, first=0):
+    scene, cam = cfg2_scene(nm, seed)
+    pos, quat = scene.walk(seed, first + n_frames, 0.1, 2.0, 15.0, 0.3)
+    out = []
+    for k in range(first, first + n_frames):
+        out.append((pos[k], quat[k], scene.render(cam, pos[k], quat[k], seed * 1000 + k)))
+    return scene, cam, out
+
+
+def frame(nm, pos, quat, res, rng, rounding, w=0, h=0, filt=0):
+    return nm.frame_from_pose(pos, quat, nm.DEFAULT_F, res, rng, HIT, MISS,
+                              rounding, w, h, filt)

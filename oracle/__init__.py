@@ -1,0 +1,192 @@
+"""Python binding of the CPU oracle (oracle/libnm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, always as the checker.  The product
+(nanomap_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnm_oracle.so")
+
+_P = ctypes.c_void_p
+_F = ctypes.c_float
+_I = ctypes.c_int
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PU32 = ctypes.POINTER(ctypes.c_uint32)
+_PF = ctypes.POINTER(ctypes.c_float)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+SIGNATURES = {
+    "or_temp_create": (_P, []),
+    "or_temp_destroy": (None, [_P]),
+    "or_temp_clear": (None, [_P]),
+    "or_cast_cloud": (_I, [_P, _P, _I, _I, _PF, _PD, _F, _F, _F, _F, _I]),
+    "or_cast_rays": (_I, [_P, _PF, _I, _PD, _F, _F, _F, _I]),
+    "or_temp_size": (ctypes.c_int64, [_P]),
+    "or_temp_export": (None, [_P, _PI32, _PU32, _PU32, _PF]),
+    "or_temp_visits": (ctypes.c_uint64, [_P]),
+    "or_map_create": (_P, [_F, _F, _F, _F]),
+    "or_map_destroy": (None, [_P]),
+    "or_map_integrate": (None, [_P, _P, _I]),
+    "or_map_leaf_count": (ctypes.c_int64, [_P]),
+    "or_map_export": (None, [_P, _PI32, _PF, _PU32]),
+    "or_map_get": (_I, [_P, _I, _I, _I, _PF, ctypes.POINTER(_I)]),
+    "or_dda_trace": (_I, [_PD, _PF, _F, _F, _I, _PI32, _I,
+                          ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "or_fold": (_F, [_F, _F, ctypes.c_uint32, _I]),
+    "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _PF, _I]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _frame_args(frame):
+    R = np.array(list(frame.rotation), np.float32)
+    eye = np.array(list(frame.eye), np.float64)
+    return R, eye
+
+
+class Temp:
+    """Temporary grid of one or more casts (Handler::populateTempGrid)."""
+
+    def __init__(self):
+        self.h = lib().or_temp_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_temp_destroy(self.h)
+            self.h = None
+
+    def cast_cloud(self, frame, cloud):
+        a = np.ascontiguousarray(cloud, dtype=np.float32)
+        a = a.reshape(a.shape[0], -1)
+        R, eye = _frame_args(frame)
+        return lib().or_cast_cloud(self.h, a.ctypes.data_as(_P), a.shape[0],
+                                   a.shape[1] * 4, _p(R, ctypes.c_float),
+                                   _p(eye, ctypes.c_double), frame.grid_res,
+                                   frame.max_time, frame.logodds_hit,
+                                   frame.logodds_miss, frame.rounding)
+
+    def cast_rays(self, frame, rays4):
+        r = np.ascontiguousarray(rays4, dtype=np.float32).reshape(-1, 4)
+        _, eye = _frame_args(frame)
+        return lib().or_cast_rays(self.h, _p(r, ctypes.c_float), r.shape[0],
+                                  _p(eye, ctypes.c_double), frame.max_time,
+                                  frame.logodds_hit, frame.logodds_miss,
+                                  frame.rounding)
+
+    def visits(self):
+        return int(lib().or_temp_visits(self.h))
+
+    def export(self):
+        """(ijk [n,3], miss [n], hit [n], value [n]) sorted by (i, j, k)."""
+        n = int(lib().or_temp_size(self.h))
+        ijk = np.zeros((max(n, 1), 3), np.int32)
+        miss = np.zeros(max(n, 1), np.uint32)
+        hit = np.zeros(max(n, 1), np.uint32)
+        val = np.zeros(max(n, 1), np.float32)
+        lib().or_temp_export(self.h, _p(ijk, ctypes.c_int32),
+                             _p(miss, ctypes.c_uint32), _p(hit, ctypes.c_uint32),
+                             _p(val, ctypes.c_float))
+        return ijk[:n], miss[:n], hit[:n], val[:n]
+
+
+class Map:
+    """Log-odds map (Handler::integrateTempGrid into map::Map)."""
+
+    def __init__(self, clamp_min=-2.0, clamp_max=3.5, hit_thr=0.0,
+                 miss_thr=0.0):
+        self.h = lib().or_map_create(clamp_min, clamp_max, hit_thr, miss_thr)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_map_destroy(self.h)
+            self.h = None
+
+    def integrate(self, temp, rounding):
+        lib().or_map_integrate(self.h, temp.h, rounding)
+
+    def leaf_count(self):
+        return int(lib().or_map_leaf_count(self.h))
+
+    def export(self):
+        L = self.leaf_count()
+        o = np.zeros((max(L, 1), 3), np.int32)
+        v = np.zeros((max(L, 1), 512), np.float32)
+        m = np.zeros((max(L, 1), 16), np.uint32)
+        lib().or_map_export(self.h, _p(o, ctypes.c_int32),
+                            _p(v, ctypes.c_float), _p(m, ctypes.c_uint32))
+        return o[:L], v[:L], m[:L]
+
+    def get(self, i, j, k):
+        v = ctypes.c_float()
+        a = ctypes.c_int()
+        ok = lib().or_map_get(self.h, i, j, k, ctypes.byref(v), ctypes.byref(a))
+        return bool(ok), v.value, bool(a.value)
+
+
+def process_frame(omap, frame, cloud):
+    """Handler::processPointCloudCPU: populate a temp grid, integrate it."""
+    t = Temp()
+    t.cast_cloud(frame, cloud)
+    omap.integrate(t, frame.rounding)
+    return t
+
+
+def dda_trace(eye, d, res, max_time, rounding, max_out=100000):
+    """Voxel sequence of one ray: ([n,3] voxels, hit flag) or None."""
+    e = np.ascontiguousarray(eye, np.float64)
+    dv = np.ascontiguousarray(d, np.float32)
+    out = np.zeros((max_out, 3), np.int32)
+    n = ctypes.c_int()
+    h = ctypes.c_int()
+    ok = lib().or_dda_trace(_p(e, ctypes.c_double), _p(dv, ctypes.c_float),
+                            res, max_time, rounding, _p(out, ctypes.c_int32),
+                            max_out, ctypes.byref(n), ctypes.byref(h))
+    if not ok:
+        return None
+    return out[:min(n.value, max_out)], bool(h.value)
+
+
+def fold(v, a, k, rounding):
+    return lib().or_fold(float(v), float(a), int(k), int(rounding))
+
+
+def voxel_filter(frame, cloud, max_rays=None):
+    a = np.ascontiguousarray(cloud, dtype=np.float32)
+    a = a.reshape(a.shape[0], -1)
+    R, eye = _frame_args(frame)
+    max_rays = max_rays or a.shape[0]
+    out = np.zeros((max(max_rays, 1), 4), np.float32)
+    n = lib().or_voxel_filter(a.ctypes.data_as(_P), a.shape[0], a.shape[1] * 4,
+                              _p(R, ctypes.c_float), _p(eye, ctypes.c_double),
+                              frame.grid_res, frame.max_time,
+                              _p(out, ctypes.c_float), max_rays)
+    return out[:min(n, max_rays)]

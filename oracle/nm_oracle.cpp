@@ -1,0 +1,512 @@
+// nm_oracle.cpp -- CPU restatement of NanoMap's frustum-sensor mapping path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path.  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Compiled with -frounding-math -ffp-contract=off so every floating-point
+// operation below is one IEEE operation in the rounding mode selected with
+// fesetround(), exactly as the reference executes it on the host.
+//
+// Parity status of each function (what pins it) is listed in the header of
+// tests/test_oracle.py and in DESIGN.md.
+
+#include "nm_oracle.h"
+
+#include <algorithm>
+#include <cfenv>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct RoundGuard {
+  int saved;
+  explicit RoundGuard(int rounding) : saved(std::fegetround()) {
+    std::fesetround(rounding == 1 ? FE_DOWNWARD : FE_TONEAREST);
+  }
+  ~RoundGuard() { std::fesetround(saved); }
+};
+
+inline uint64_t leaf_key(int i, int j, int k) {
+  // Leaf origin coordinates (voxel >> 3) packed in 21 bits each.
+  const uint64_t m = (1ull << 21) - 1;
+  return ((uint64_t)((i >> 3) & m) << 42) | ((uint64_t)((j >> 3) & m) << 21) |
+         (uint64_t)((k >> 3) & m);
+}
+
+inline int local_index(int i, int j, int k) {
+  // In-leaf offset order of the reference: z + 8y + 64x
+  // (frustumCastCloud.cu:215, Handler.cpp:171).
+  return (k & 7) + ((j & 7) << 3) + ((i & 7) << 6);
+}
+
+struct TempLeaf {
+  int origin[3];
+  float v[512];
+  uint32_t miss[512];
+  uint32_t hit[512];
+  uint64_t on[8];
+};
+
+struct MapLeaf {
+  int origin[3];
+  float v[512];
+  uint64_t active[8];
+};
+
+template <class LeafT>
+struct SparseGrid {
+  std::unordered_map<uint64_t, LeafT*> leaves;
+  uint64_t last_key = ~0ull;
+  LeafT* last = nullptr;
+
+  ~SparseGrid() { clear(); }
+  void clear() {
+    for (auto& kv : leaves) delete kv.second;
+    leaves.clear();
+    last_key = ~0ull;
+    last = nullptr;
+  }
+  // Accessor-style lookup with a one-leaf cache (the cost model of
+  // openvdb::tree::ValueAccessor).
+  LeafT* probe(int i, int j, int k) const {
+    const uint64_t key = leaf_key(i, j, k);
+    if (key == last_key) return last;
+    auto it = leaves.find(key);
+    return it == leaves.end() ? nullptr : it->second;
+  }
+  LeafT* touch(int i, int j, int k) {
+    const uint64_t key = leaf_key(i, j, k);
+    if (key == last_key) return last;
+    auto it = leaves.find(key);
+    LeafT* leaf;
+    if (it == leaves.end()) {
+      leaf = new LeafT();
+      std::memset(leaf, 0, sizeof(LeafT));
+      leaf->origin[0] = i & ~7;
+      leaf->origin[1] = j & ~7;
+      leaf->origin[2] = k & ~7;
+      leaves.emplace(key, leaf);
+    } else {
+      leaf = it->second;
+    }
+    last_key = key;
+    last = leaf;
+    return leaf;
+  }
+};
+
+}  // namespace
+
+struct or_temp {
+  SparseGrid<TempLeaf> grid;
+  uint64_t visits = 0;
+};
+
+struct or_map {
+  SparseGrid<MapLeaf> grid;
+  float clamp_min, clamp_max, hit_thr, miss_thr;
+};
+
+namespace {
+
+// openvdb::math::MinIndex: x only if strictly smallest, else y if y < z,
+// else z (ties resolve toward the higher axis).
+inline int min_index(const double n[3]) {
+  if (n[0] < n[1] && n[0] < n[2]) return 0;
+  if (n[1] < n[2]) return 1;
+  return 2;
+}
+
+// One ray of Handler::populateTempGrid (Handler.cpp:65-84) after the
+// direction has been formed: openvdb::math::DDA<Ray<double>,0> from eye
+// over [0, ray_length]; a miss on every voxel stepped out of, a hit on the
+// end voxel iff its exit time < max_time.
+template <class Visit>
+inline void dda_cast(const double eye[3], double dx, double dy, double dz,
+                     double ray_length, float max_time, Visit&& visit) {
+  // Ray::setDir (openvdb math/Ray.h): inv = 1 / dir.
+  const double dir[3] = {dx, dy, dz};
+  double inv[3];
+  for (int a = 0; a < 3; ++a) inv[a] = 1 / dir[a];
+  // DDA::init(ray, 0, ray_length): pos = ray(0) = eye + dir * 0.
+  double pos[3];
+  for (int a = 0; a < 3; ++a) pos[a] = eye[a] + dir[a] * 0.0;
+  int voxel[3];
+  for (int a = 0; a < 3; ++a) voxel[a] = (int)std::floor(pos[a]);
+  int step[3];
+  double next[3], delta[3];
+  const double t0 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    if (dir[a] == 0.0) {  // math::isZero: exact comparison
+      step[a] = 0;
+      next[a] = std::numeric_limits<double>::max();
+      delta[a] = std::numeric_limits<double>::max();
+    } else if (inv[a] > 0) {
+      step[a] = 1;
+      next[a] = t0 + ((double)(voxel[a] + 1) - pos[a]) * inv[a];
+      delta[a] = (double)step[a] * inv[a];
+    } else {
+      step[a] = -1;
+      next[a] = t0 + ((double)(voxel[a]) - pos[a]) * inv[a];
+      delta[a] = (double)step[a] * inv[a];
+    }
+  }
+  int prev[3] = {voxel[0], voxel[1], voxel[2]};
+  double t;
+  for (;;) {
+    // DDA::step()
+    const int a = min_index(next);
+    t = next[a];
+    next[a] += delta[a];
+    voxel[a] += step[a];
+    if (!(t <= ray_length)) break;
+    visit(prev, false);
+    prev[0] = voxel[0];
+    prev[1] = voxel[1];
+    prev[2] = voxel[2];
+  }
+  if (t < (double)max_time) visit(prev, true);
+}
+
+// Direction set-up of Handler::populateTempGrid (Handler.cpp:61-75).
+// Returns false when the pixel is skipped (ray_length >= max_time or NaN).
+inline bool ray_setup(const float R[9], float px, float py, float pz,
+                      float res, float max_time, double* d, double* L) {
+  // Eigen float products, evaluated left to right (Handler.cpp:65-68).
+  const float fx = R[0] * px + R[1] * py + R[2] * pz;
+  const float fy = R[3] * px + R[4] * py + R[5] * pz;
+  const float fz = R[6] * px + R[7] * py + R[8] * pz;
+  double x = fx, y = fy, z = fz;
+  // Vec3d::length() / gridRes (Handler.cpp:70).
+  const double len = std::sqrt(x * x + y * y + z * z);
+  const double ray_length = len / (double)res;
+  if (!(ray_length < (double)max_time)) return false;  // Handler.cpp:71
+  // Vec3d::normalize(eps = 1e-7): skip if |len - 0| <= eps.
+  if (!(std::fabs(len - 0.0) <= 1.0e-7)) {
+    const double s = 1 / len;
+    x *= s;
+    y *= s;
+    z *= s;
+  }
+  d[0] = x;
+  d[1] = y;
+  d[2] = z;
+  *L = ray_length;
+  return true;
+}
+
+inline void temp_visit(or_temp* t, const int v[3], bool is_hit, float hit,
+                       float miss) {
+  TempLeaf* leaf = t->grid.touch(v[0], v[1], v[2]);
+  const int li = local_index(v[0], v[1], v[2]);
+  // modifyValueAndActiveState(voxel, miss/hit): value += logodds; on.
+  leaf->v[li] += is_hit ? hit : miss;
+  if (is_hit)
+    leaf->hit[li] += 1;
+  else
+    leaf->miss[li] += 1;
+  leaf->on[li >> 6] |= 1ull << (li & 63);
+  t->visits += 1;
+}
+
+template <class LeafT>
+std::vector<const LeafT*> sorted_leaves(const SparseGrid<LeafT>& g) {
+  std::vector<const LeafT*> out;
+  out.reserve(g.leaves.size());
+  for (auto& kv : g.leaves) out.push_back(kv.second);
+  std::sort(out.begin(), out.end(), [](const LeafT* a, const LeafT* b) {
+    if (a->origin[0] != b->origin[0]) return a->origin[0] < b->origin[0];
+    if (a->origin[1] != b->origin[1]) return a->origin[1] < b->origin[1];
+    return a->origin[2] < b->origin[2];
+  });
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+or_temp* or_temp_create(void) { return new or_temp(); }
+void or_temp_destroy(or_temp* t) { delete t; }
+void or_temp_clear(or_temp* t) {
+  t->grid.clear();
+  t->visits = 0;
+}
+
+// Handler::populateTempGrid (Handler.cpp:28-86), with the hit/miss
+// log-odds passed in directly (the reference derives the hit value from
+// _probMiss by mistake, Handler.cpp:43; SimHandler.cpp:42 is correct).
+int or_cast_cloud(or_temp* t, const unsigned char* cloud, int n_points,
+                  int step, const float R[9], const double eye[3], float res,
+                  float max_time, float logodds_hit, float logodds_miss,
+                  int rounding) {
+  RoundGuard guard(rounding);
+  int rays = 0;
+  for (int i = 0; i < n_points; ++i) {
+    const unsigned char* p = cloud + (size_t)i * step;
+    float px, py, pz;
+    std::memcpy(&px, p + 0, 4);
+    std::memcpy(&py, p + 4, 4);
+    std::memcpy(&pz, p + 8, 4);
+    double d[3], L;
+    if (!ray_setup(R, px, py, pz, res, max_time, d, &L)) continue;
+    ++rays;
+    dda_cast(eye, d[0], d[1], d[2], L, max_time,
+             [&](const int v[3], bool is_hit) {
+               temp_visit(t, v, is_hit, logodds_hit, logodds_miss);
+             });
+  }
+  return rays;
+}
+
+int or_cast_rays(or_temp* t, const float* rays4, int n_rays,
+                 const double eye[3], float max_time, float logodds_hit,
+                 float logodds_miss, int rounding) {
+  RoundGuard guard(rounding);
+  int rays = 0;
+  for (int i = 0; i < n_rays; ++i) {
+    const float* r = rays4 + 4 * (size_t)i;
+    const double L = r[3];
+    if (!(L > 0.0 && L < (double)max_time)) continue;
+    ++rays;
+    dda_cast(eye, (double)r[0], (double)r[1], (double)r[2], L, max_time,
+             [&](const int v[3], bool is_hit) {
+               temp_visit(t, v, is_hit, logodds_hit, logodds_miss);
+             });
+  }
+  return rays;
+}
+
+int64_t or_temp_size(const or_temp* t) {
+  int64_t n = 0;
+  for (auto& kv : t->grid.leaves)
+    for (int w = 0; w < 8; ++w) n += __builtin_popcountll(kv.second->on[w]);
+  return n;
+}
+
+uint64_t or_temp_visits(const or_temp* t) { return t->visits; }
+
+void or_temp_export(const or_temp* t, int32_t* ijk, uint32_t* miss,
+                    uint32_t* hit, float* value) {
+  int64_t n = 0;
+  for (const TempLeaf* leaf : sorted_leaves(t->grid)) {
+    // local index li = z + 8y + 64x ascends in (x, y, z) lexicographic order
+    for (int li = 0; li < 512; ++li) {
+      if (!((leaf->on[li >> 6] >> (li & 63)) & 1)) continue;
+      if (ijk) {
+        ijk[3 * n + 0] = leaf->origin[0] + (li >> 6);
+        ijk[3 * n + 1] = leaf->origin[1] + ((li >> 3) & 7);
+        ijk[3 * n + 2] = leaf->origin[2] + (li & 7);
+      }
+      if (miss) miss[n] = leaf->miss[li];
+      if (hit) hit[n] = leaf->hit[li];
+      if (value) value[n] = leaf->v[li];
+      ++n;
+    }
+  }
+}
+
+or_map* or_map_create(float clamp_min, float clamp_max, float hit_thr,
+                      float miss_thr) {
+  or_map* m = new or_map();
+  m->clamp_min = clamp_min;
+  m->clamp_max = clamp_max;
+  m->hit_thr = hit_thr;
+  m->miss_thr = miss_thr;
+  return m;
+}
+
+void or_map_destroy(or_map* m) { delete m; }
+
+// Handler::integrateTempGrid (Handler.cpp:88-122): for each active temp
+// voxel with a nonzero value: m += tv, clamp to [empty, occ], then active =
+// true if m > hitThr, false if m < missThr, unchanged otherwise.
+void or_map_integrate(or_map* m, const or_temp* t, int rounding) {
+  RoundGuard guard(rounding);
+  for (const TempLeaf* tl : sorted_leaves(t->grid)) {
+    for (int li = 0; li < 512; ++li) {
+      if (!((tl->on[li >> 6] >> (li & 63)) & 1)) continue;
+      const float tv = tl->v[li];
+      if (!(tv != 0.0f)) continue;
+      const int i = tl->origin[0] + (li >> 6);
+      const int j = tl->origin[1] + ((li >> 3) & 7);
+      const int k = tl->origin[2] + (li & 7);
+      MapLeaf* ml = m->grid.touch(i, j, k);
+      float value = ml->v[li];
+      bool active = (ml->active[li >> 6] >> (li & 63)) & 1;
+      value += tv;
+      if (value > m->clamp_max) {
+        value = m->clamp_max;
+      } else if (value < m->clamp_min) {
+        value = m->clamp_min;
+      }
+      if (value > m->hit_thr) {
+        active = true;
+      } else if (value < m->miss_thr) {
+        active = false;
+      }
+      ml->v[li] = value;
+      if (active)
+        ml->active[li >> 6] |= 1ull << (li & 63);
+      else
+        ml->active[li >> 6] &= ~(1ull << (li & 63));
+    }
+  }
+}
+
+int64_t or_map_leaf_count(const or_map* m) {
+  return (int64_t)m->grid.leaves.size();
+}
+
+void or_map_export(const or_map* m, int32_t* origins, float* values,
+                   uint32_t* masks) {
+  int64_t n = 0;
+  for (const MapLeaf* leaf : sorted_leaves(m->grid)) {
+    if (origins) std::memcpy(origins + 3 * n, leaf->origin, 12);
+    if (values) std::memcpy(values + 512 * n, leaf->v, 2048);
+    if (masks)
+      for (int w = 0; w < 16; ++w)
+        masks[16 * n + w] = (uint32_t)(leaf->active[w >> 1] >> (32 * (w & 1)));
+    ++n;
+  }
+}
+
+int or_map_get(const or_map* m, int i, int j, int k, float* value,
+               int* active) {
+  const MapLeaf* leaf = m->grid.probe(i, j, k);
+  if (!leaf) {
+    *value = 0.0f;
+    *active = 0;
+    return 0;
+  }
+  const int li = local_index(i, j, k);
+  *value = leaf->v[li];
+  *active = (int)((leaf->active[li >> 6] >> (li & 63)) & 1);
+  return 1;
+}
+
+int or_dda_trace(const double eye[3], const float d[3], float res,
+                 float max_time, int rounding, int32_t* out_ijk, int max_out,
+                 int* n_out, int* hit) {
+  RoundGuard guard(rounding);
+  const float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double dir[3], L;
+  *n_out = 0;
+  *hit = 0;
+  if (!ray_setup(R, d[0], d[1], d[2], res, max_time, dir, &L)) return 0;
+  int n = 0;
+  dda_cast(eye, dir[0], dir[1], dir[2], L, max_time,
+           [&](const int v[3], bool is_hit) {
+             if (n < max_out) {
+               out_ijk[3 * n + 0] = v[0];
+               out_ijk[3 * n + 1] = v[1];
+               out_ijk[3 * n + 2] = v[2];
+             }
+             ++n;
+             if (is_hit) *hit = 1;
+           });
+  *n_out = n;
+  return 1;
+}
+
+float or_fold(float v0, float a, uint32_t k, int rounding) {
+  RoundGuard guard(rounding);
+  volatile float v = v0;
+  for (uint32_t i = 0; i < k; ++i) v = v + a;
+  return v;
+}
+
+// Deterministic restatement of voxelFilter + rayFromFilter
+// (filterCloud.cu:84-133, 279-318), FP32 in round-to-nearest (the
+// reference runs it on the GPU).  Points are binned by
+// floor(r/res + voxelOriginOffset), r = R p; per bin the fractions
+// frac(r/res) are summed in ascending pixel order; each bin emits the ray
+// (mean + bin) normalised with length |mean + bin| if 0 < length < max_time.
+// Validity: no NaN coordinate and 0 < |p| < max range (max_time * res).
+// Bins are relative to floor(eye) and emitted in ascending
+// (x, y, z) order.
+int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
+                    const float R[9], const double eye[3], float res,
+                    float max_time, float* rays4, int max_rays) {
+  RoundGuard guard(0);
+  struct Bin {
+    float sx, sy, sz;
+    uint32_t n;
+  };
+  const float max_range = max_time * res;
+  float origin[3];
+  for (int a = 0; a < 3; ++a)
+    origin[a] = (float)(eye[a] - std::floor(eye[a]));
+  struct Key {
+    int x, y, z;
+    bool operator<(const Key& o) const {
+      if (x != o.x) return x < o.x;
+      if (y != o.y) return y < o.y;
+      return z < o.z;
+    }
+  };
+  std::vector<std::pair<Key, Bin>> bins;
+  std::unordered_map<uint64_t, size_t> index;
+  for (int i = 0; i < n_points; ++i) {
+    const unsigned char* p = cloud + (size_t)i * step;
+    float px, py, pz;
+    std::memcpy(&px, p + 0, 4);
+    std::memcpy(&py, p + 4, 4);
+    std::memcpy(&pz, p + 8, 4);
+    if (std::isnan(px) || std::isnan(py) || std::isnan(pz)) continue;
+    const float norm = std::sqrt(px * px + py * py + pz * pz);
+    if (!(norm > 0.0f && norm < max_range)) continue;
+    const float r[3] = {R[0] * px + R[1] * py + R[2] * pz,
+                        R[3] * px + R[4] * py + R[5] * pz,
+                        R[6] * px + R[7] * py + R[8] * pz};
+    int b[3];
+    float f[3];
+    for (int a = 0; a < 3; ++a) {
+      const float q = r[a] / res;
+      b[a] = (int)std::floor(q + origin[a]);
+      f[a] = q - (float)(int)std::floor(q);
+    }
+    const uint64_t key = ((uint64_t)(uint32_t)(b[0] + (1 << 20)) << 42) |
+                         ((uint64_t)(uint32_t)(b[1] + (1 << 20)) << 21) |
+                         (uint64_t)(uint32_t)(b[2] + (1 << 20));
+    auto it = index.find(key);
+    if (it == index.end()) {
+      index.emplace(key, bins.size());
+      bins.push_back({Key{b[0], b[1], b[2]}, Bin{f[0], f[1], f[2], 1u}});
+    } else {
+      Bin& bin = bins[it->second].second;
+      bin.sx += f[0];
+      bin.sy += f[1];
+      bin.sz += f[2];
+      bin.n += 1;
+    }
+  }
+  std::sort(bins.begin(), bins.end(),
+            [](const std::pair<Key, Bin>& a, const std::pair<Key, Bin>& b) {
+              return a.first < b.first;
+            });
+  int n = 0;
+  for (const auto& kb : bins) {
+    const float cnt = (float)kb.second.n;
+    float pt[3] = {kb.second.sx / cnt + (float)kb.first.x,
+                   kb.second.sy / cnt + (float)kb.first.y,
+                   kb.second.sz / cnt + (float)kb.first.z};
+    const float len = std::sqrt(pt[0] * pt[0] + pt[1] * pt[1] + pt[2] * pt[2]);
+    if (!(len > 0.0f && len < max_time)) continue;
+    if (n < max_rays) {
+      rays4[4 * n + 0] = pt[0] / len;
+      rays4[4 * n + 1] = pt[1] / len;
+      rays4[4 * n + 2] = pt[2] / len;
+      rays4[4 * n + 3] = len;
+    }
+    ++n;
+  }
+  return n;
+}
+
+}  // extern "C"

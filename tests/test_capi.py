@@ -1,0 +1,120 @@
+"""CPU tests of the C ABI: the library loads, exports every symbol the
+public headers declare, and its host-only functions agree with the oracle.
+No device calls here."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(nm_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_entry_points():
+    names = declared_symbols()
+    for n in ("nm_ctx_create", "nm_cast", "nm_update_map", "nm_frame_submit",
+              "nm_map_create", "nm_map_download", "nm_scene_render"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(nm):
+    L = ctypes.CDLL(nm._capi.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding declares a signature for each of them
+    from nanomap_amd._capi import SIGNATURES
+    assert not [n for n in declared_symbols() if n not in SIGNATURES]
+
+
+def test_fold_host_matches_oracle_sequential_sums(nm, oracle_lib):
+    rng = np.random.default_rng(21)
+    cases = [(0.0, -0.4, k) for k in (0, 1, 2, 7, 100, 4096, 300000, 921600)]
+    cases += [(0.0, 0.85, k) for k in (1, 5, 3000, 100000)]
+    for _ in range(300):
+        v = float(np.float32(rng.uniform(-200, 200)))
+        a = float(np.float32(rng.choice([-0.4, 0.85, rng.uniform(-2, 2)])))
+        cases.append((v, a, int(rng.integers(0, 20000))))
+    for mode in (0, 1):
+        for v, a, k in cases:
+            g = np.float32(nm.fold(v, a, k, mode)).view(np.uint32)
+            o = np.float32(oracle.fold(v, a, k, mode)).view(np.uint32)
+            assert g == o, (mode, v, a, k)
+
+
+def test_frame_from_pose_closed_forms(nm):
+    # identity quaternion, identity frame transform -> R = I, eye = p / res
+    I = np.eye(3, dtype=np.float32)
+    f = nm.frame_from_pose([1.0, -2.0, 0.5], [1, 0, 0, 0], I, 0.1, 5.0,
+                           0.85, -0.4, nm.ROUND_NEAREST)
+    assert list(f.rotation) == [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    inv = 1.0 / float(np.float32(0.1))
+    assert list(f.eye) == [float(np.float32(1.0)) * inv,
+                           float(np.float32(-2.0)) * inv, 0.5 * inv]
+    assert f.max_time == np.float32(np.float32(5.0) / np.float32(0.1))
+    # 90 degrees about z: exact permutation matrix
+    s = np.float32(np.sqrt(0.5))
+    f = nm.frame_from_pose([0, 0, 0], [s, 0, 0, s], I, 0.1, 5.0, 0.85, -0.4, 0)
+    R = np.array(f.rotation).reshape(3, 3)
+    assert np.allclose(R, [[0, -1, 0], [1, 0, 0], [0, 0, 1]], atol=1e-6)
+    # sample config frame transform maps optical z to body x
+    f = nm.frame_from_pose([0, 0, 0], [1, 0, 0, 0], nm.DEFAULT_F, 0.1, 5.0,
+                           0.85, -0.4, 0)
+    R = np.array(f.rotation).reshape(3, 3)
+    assert np.array_equal(R @ [0, 0, 1], [1, 0, 0])
+    # round-down mode: max_time = RD(5 / 0.1f) < RN(5 / 0.1f)
+    f_rd = nm.frame_from_pose([0, 0, 0], [1, 0, 0, 0], I, 0.1, 5.0, 0.85, -0.4, 1)
+    f_rn = nm.frame_from_pose([0, 0, 0], [1, 0, 0, 0], I, 0.1, 5.0, 0.85, -0.4, 0)
+    assert f_rd.max_time < f_rn.max_time
+
+
+def test_scene_is_deterministic_and_plausible(nm):
+    from nanomap_amd import workloads
+    _, cam, frames = workloads.cfg1(nm, seed=1)
+    _, _, frames2 = workloads.cfg1(nm, seed=1)
+    a, b = frames[0][2], frames2[0][2]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    z = a[:, 2]
+    assert np.all(z > 0) and np.all(z <= 5.0)
+    # a good share of pixels sees a surface inside the 5 m cap
+    assert np.mean(z < 5.0) > 0.1
+    _, _, frames3 = workloads.cfg1(nm, seed=2)
+    assert not np.array_equal(a, frames3[0][2])
+
+
+def test_dropouts_and_noise(nm):
+    scene = nm.Scene(3, (10.0, 10.0, 4.0), 5)
+    pos, quat = scene.sample_pose(3)
+    cam = nm.camera(64, 48, 80.0, 60.0, 8.0, noise_k=0.0, dropout=0.25)
+    c = scene.render(cam, pos, quat, 7)
+    frac = np.mean(np.isnan(c[:, 0]))
+    assert 0.15 < frac < 0.35
+    cam0 = nm.camera(64, 48, 80.0, 60.0, 8.0, noise_k=0.0, dropout=0.0)
+    camn = nm.camera(64, 48, 80.0, 60.0, 8.0, noise_k=0.002, dropout=0.0)
+    c0 = scene.render(cam0, pos, quat, 7)
+    cn = scene.render(camn, pos, quat, 7)
+    d = np.abs(cn[:, 2] - c0[:, 2])
+    ok = c0[:, 2] < 7.9
+    assert np.all(d[ok] < 0.002 * c0[ok, 2] ** 2 * 6 + 1e-6)
+    assert np.mean(d[ok] > 0) > 0.9
+
+
+def test_context_needs_device(nm):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("device present: covered by gpu tests")
+    with pytest.raises(nm.NanoMapError):
+        nm.Context(100, 50.0)

@@ -1,0 +1,243 @@
+"""GPU parity of the plain frame path against the CPU oracle.
+
+Each test runs the HIP path through the C ABI (nanomap_amd -> libnanomap_amd
+.so) and the oracle (Handler::populateTempGrid + integrateTempGrid
+restated in oracle/) on identical clouds, and requires bit-exact results:
+per-voxel (miss, hit) counts of every frame, and the map (leaf set, FP32
+log-odds bit patterns, active masks).  Both rounding modes are covered;
+round-down is what the reference executes.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from nanomap_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+RN, RD = 0, 1
+
+
+def _check_counts(ctx, temp):
+    g_ijk, g_m, g_h = ctx.counts()
+    o_ijk, o_m, o_h, _ = temp.export()
+    assert g_ijk.shape == o_ijk.shape, (g_ijk.shape, o_ijk.shape)
+    assert np.array_equal(g_ijk, o_ijk)
+    assert np.array_equal(g_m, o_m)
+    assert np.array_equal(g_h, o_h)
+
+
+def _check_maps(gmap, omap):
+    go, gv, gm = gmap.download()
+    oo, ov, om = omap.export()
+    assert go.shape == oo.shape, (go.shape, oo.shape)
+    assert np.array_equal(go, oo)
+    bad = np.nonzero(gv.view(np.uint32) != ov.view(np.uint32))
+    assert bad[0].size == 0, (bad[0][:5], bad[1][:5], gv[bad][:5], ov[bad][:5])
+    assert np.array_equal(gm, om)
+
+
+def _run_frame(nm, ctx, gmap, omap, frame, cloud, check_counts=True):
+    print("frame: points=%d res=%g max_time=%g rounding=%d filter=%d"
+          % (len(cloud), frame.grid_res, frame.max_time, frame.rounding,
+             frame.filter), flush=True)
+    ctx.cast(frame, cloud)
+    temp = oracle.Temp()
+    if frame.filter == nm.FILTER_VOXEL:
+        rays = oracle.voxel_filter(frame, cloud)
+        temp.cast_rays(frame, rays)
+    else:
+        temp.cast_cloud(frame, cloud)
+    if check_counts:
+        _check_counts(ctx, temp)
+    ctx.update(gmap)
+    omap.integrate(temp, frame.rounding)
+    return temp
+
+
+@pytest.mark.parametrize("rounding", [RD, RN])
+def test_cfg1_counts_and_map_bit_exact(gpu, rounding):
+    nm = gpu
+    _, cam, frames = W.cfg1(nm)
+    pos, quat, cloud = frames[0]
+    f = W.frame(nm, pos, quat, 0.1, 5.0, rounding, cam.width, cam.height)
+    ctx = nm.Context(cam.width * cam.height, f.max_time)
+    gmap = nm.Map(0.1, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    temp = _run_frame(nm, ctx, gmap, omap, f, cloud)
+    st = ctx.stats()
+    _, m, h, _ = temp.export()
+    assert st["errors"] == 0 and st["overflow"] == 0
+    assert st["visits"] == temp.visits() == int(m.sum() + h.sum())
+    assert st["voxels"] == len(m)
+    assert st["hit_voxels"] == int((h > 0).sum())
+    assert st["mixed_voxels"] == int(((h > 0) & (m > 0)).sum())
+    _check_maps(gmap, omap)
+
+
+def test_cfg2_ten_frames_bit_exact(gpu):
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=10)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, cam.width, cam.height)
+        _run_frame(nm, ctx, gmap, omap, f, cloud)
+        assert ctx.stats()["errors"] == 0
+    _check_maps(gmap, omap)
+
+
+@pytest.mark.timeout(900)
+def test_cfg2_hundred_frames_bit_exact(gpu):
+    """100-frame 640x480 sequence at 0.05 m; every
+    frame's counts and the final map bit-exact versus the CPU path."""
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=100)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, cam.width, cam.height)
+        _run_frame(nm, ctx, gmap, omap, f, cloud)
+    st = ctx.stats()
+    assert st["errors"] == 0 and st["overflow"] == 0
+    _check_maps(gmap, omap)
+
+
+def test_edge_cases(gpu):
+    nm = gpu
+    I = np.eye(3, dtype=np.float32)
+    ctx = nm.Context(4096, 100.0)
+    gmap = nm.Map(0.1, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    rng = np.random.default_rng(0)
+    cases = {
+        "empty": np.zeros((0, 3), np.float32),
+        "all_nan": np.full((50, 3), np.nan, np.float32),
+        "single": np.array([[0.3, 0.2, 1.1]], np.float32),
+        "zero_point": np.array([[0, 0, 0], [0.5, 0, 0]], np.float32),
+        "beyond_range": np.array([[20.0, 0, 0], [0, 9.99, 0]], np.float32),
+        "axis_and_diagonal": np.array([[1, 0, 0], [0, -1, 0], [0, 0, 1],
+                                       [0.4, 0.4, 0.4], [-0.4, -0.4, -0.4]],
+                                      np.float32),
+        "random": (rng.normal(size=(3000, 3)) * 3).astype(np.float32),
+    }
+    # integer eye (rays start on voxel corners: ties at t = 0) and a
+    # position exactly 0.0
+    poses = [([0.0, 0.0, 0.0], [1, 0, 0, 0]), ([1.0, 2.0, 3.0], [1, 0, 0, 0]),
+             ([-3.05, 7.33, 0.71], [0.9, 0.1, -0.3, 0.2])]
+    for rounding in (RD, RN):
+        for pos, q in poses:
+            for name, cloud in cases.items():
+                f = nm.frame_from_pose(pos, q, I, 0.1, 10.0, W.HIT, W.MISS,
+                                       rounding)
+                print("case", name, pos, rounding, flush=True)
+                _run_frame(nm, ctx, gmap, omap, f, cloud)
+                assert ctx.stats()["errors"] == 0
+    _check_maps(gmap, omap)
+
+
+def test_stride_16_and_device_pointer(gpu):
+    import torch
+    nm = gpu
+    _, cam, frames = W.cfg1(nm)
+    pos, quat, cloud = frames[0]
+    f = W.frame(nm, pos, quat, 0.1, 5.0, RD)
+    padded = np.concatenate([cloud, np.full((len(cloud), 1), 7.0, np.float32)], 1)
+    ctx = nm.Context(len(cloud), f.max_time)
+    g1 = nm.Map(0.1, *W.CLAMP)
+    ctx.submit(g1, f, padded)             # host pointer, stride 16
+    dev = torch.from_numpy(padded).cuda()
+    g2 = nm.Map(0.1, *W.CLAMP)
+    ctx.submit(g2, f, dev)                # device pointer, stride 16
+    torch.cuda.synchronize()
+    omap = oracle.Map(*W.CLAMP)
+    oracle.process_frame(omap, f, cloud)
+    _check_maps(g1, omap)
+    _check_maps(g2, omap)
+
+
+def test_run_twice_is_identical(gpu):
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=1)
+    pos, quat, cloud = frames[0]
+    f = W.frame(nm, pos, quat, 0.05, 10.0, RD)
+    ctx = nm.Context(len(cloud), f.max_time)
+    outs = []
+    for _ in range(2):
+        g = nm.Map(0.05, *W.CLAMP)
+        ctx.submit(g, f, cloud)
+        o, v, m = g.download()
+        outs.append((o, v.view(np.uint32), m))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("rounding", [RD, RN])
+def test_voxel_filter_bit_exact(gpu, rounding):
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, rounding, filt=nm.FILTER_VOXEL)
+        ctx.cast(f, cloud)
+        g_rays = ctx.filter_rays()
+        o_rays = oracle.voxel_filter(f, cloud)
+        assert g_rays.shape == o_rays.shape and len(o_rays) > 1000
+        assert np.array_equal(g_rays.view(np.uint32), o_rays.view(np.uint32))
+        temp = oracle.Temp()
+        temp.cast_rays(f, o_rays)
+        _check_counts(ctx, temp)
+        ctx.update(gmap)
+        omap.integrate(temp, rounding)
+    _check_maps(gmap, omap)
+
+
+def test_overflow_is_reported_not_fatal(gpu):
+    nm = gpu
+    _, cam, frames = W.cfg1(nm)
+    pos, quat, cloud = frames[0]
+    f = W.frame(nm, pos, quat, 0.1, 5.0, RD)
+    ctx = nm.Context(len(cloud), f.max_time, tile_capacity=4)
+    g = nm.Map(0.1, *W.CLAMP)
+    ctx.submit(g, f, cloud)
+    st = ctx.stats()
+    assert st["overflow"] > 0
+    # the context stays usable and a big enough map/context is exact
+    small_map = nm.Map(0.1, *W.CLAMP, leaf_capacity=2)
+    ctx2 = nm.Context(len(cloud), f.max_time)
+    ctx2.submit(small_map, f, cloud)
+    assert ctx2.stats()["overflow"] > 0
+    assert small_map.leaf_count() == 2
+    g2 = nm.Map(0.1, *W.CLAMP)
+    ctx.close()
+    ctx2.submit(g2, f, cloud)
+    omap = oracle.Map(*W.CLAMP)
+    oracle.process_frame(omap, f, cloud)
+    _check_maps(g2, omap)
+
+
+def test_bad_arguments_raise(gpu):
+    nm = gpu
+    ctx = nm.Context(100, 50.0)
+    g = nm.Map(0.1)
+    f = nm.frame_from_pose([0, 0, 0], [1, 0, 0, 0], np.eye(3, dtype=np.float32),
+                           0.1, 5.0, 0.85, -0.4, RD)
+    with pytest.raises(nm.NanoMapError):
+        ctx.cast(f, np.zeros((101, 3), np.float32))  # too many points
+    big = f.copy()
+    big.max_time = 60.0
+    with pytest.raises(nm.NanoMapError):
+        ctx.cast(big, np.zeros((10, 3), np.float32))  # beyond context range
+    with pytest.raises(nm.NanoMapError) as e:
+        ctx.update(g)  # no pending cast
+    assert e.value.code == 6
+    ctx.cast(f, np.ones((10, 3), np.float32))
+    with pytest.raises(nm.NanoMapError):
+        ctx.cast(f, np.ones((10, 3), np.float32))  # previous cast not folded
+    ctx.update(g)
