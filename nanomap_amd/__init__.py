@@ -12,9 +12,10 @@ import numpy as np
 
 from ._capi import (FILTER_NONE, FILTER_VOXEL, ROUND_DOWN, ROUND_NEAREST,
                     Camera, CtxConfig, Frame, FrameStats, NanoMapError,
-                    _ptr, check, lib)
+                    SimSensor, _ptr, check, lib)
 
 __all__ = ["Context", "Map", "Scene", "Camera", "Frame", "FrameStats",
+           "SimGrid", "SimSensor", "sim_sensor",
            "frame_from_pose", "fold", "NanoMapError", "ROUND_DOWN",
            "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "DEFAULT_F"]
 
@@ -274,3 +275,76 @@ def camera(width, height, hfov_deg, vfov_deg, max_range, noise_k=0.002,
     for i, x in enumerate(np.asarray(F, np.float32).reshape(9)):
         c.frame[i] = float(x)
     return c
+
+
+def sim_sensor(pos, rotation, grid_res, width, height, hfov_deg, vfov_deg,
+               min_range, max_range):
+    s = SimSensor()
+    for k in range(3):
+        s.position[k] = float(pos[k])
+    for k, x in enumerate(np.asarray(rotation, np.float32).reshape(9)):
+        s.rotation[k] = float(x)
+    s.grid_res = grid_res
+    s.width, s.height = width, height
+    s.hfov_deg, s.vfov_deg = hfov_deg, vfov_deg
+    s.min_range, s.max_range = min_range, max_range
+    return s
+
+
+class SimGrid:
+    """Device occupancy grid for the sensor-simulation cast (generateCloud)."""
+
+    def __init__(self, origins=None, values=None, threshold=0.0, device=0,
+                 from_map=None):
+        h = ctypes.c_void_p()
+        if from_map is not None:
+            check(lib().nm_simgrid_from_map(from_map.handle, threshold,
+                                            ctypes.byref(h)))
+        else:
+            o = np.ascontiguousarray(origins, np.int32).reshape(-1, 3)
+            v = np.ascontiguousarray(values, np.float32).reshape(-1, 512)
+            check(lib().nm_simgrid_from_leaves(device, _ptr(o, ctypes.c_int32),
+                                               _ptr(v, ctypes.c_float),
+                                               o.shape[0], threshold,
+                                               ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().nm_simgrid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        a = ctypes.c_int64()
+        bb = np.zeros(6, np.int32)
+        cn = np.zeros(3, np.int32)
+        check(lib().nm_simgrid_info(self.handle, ctypes.byref(a),
+                                    _ptr(bb, ctypes.c_int32),
+                                    _ptr(cn, ctypes.c_int32)))
+        return {"active": a.value, "bbox": bb.tolist(), "nodes": cn.tolist()}
+
+    def cast(self, sensors, stream=None):
+        """Simulate the sensors; returns device torch tensors
+        (ijk [N,3] int32, dist [N] f32, xyz [N,3] f32) and the return
+        count, N = total pixels."""
+        import torch
+        if isinstance(sensors, SimSensor):
+            sensors = [sensors]
+        arr = (SimSensor * len(sensors))(*sensors)
+        N = sum(s.width * s.height for s in sensors)
+        ijk = torch.empty((N, 3), dtype=torch.int32, device="cuda")
+        dist = torch.empty(N, dtype=torch.float32, device="cuda")
+        xyz = torch.empty((N, 3), dtype=torch.float32, device="cuda")
+        n = ctypes.c_int()
+        check(lib().nm_sim_cast(self.handle, arr, len(sensors),
+                                ctypes.c_void_p(ijk.data_ptr()),
+                                ctypes.c_void_p(dist.data_ptr()),
+                                ctypes.c_void_p(xyz.data_ptr()),
+                                ctypes.byref(n), stream))
+        return ijk, dist, xyz, n.value

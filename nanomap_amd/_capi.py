@@ -63,6 +63,14 @@ class FrameStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SimSensor(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3),
+                ("rotation", ctypes.c_float * 9), ("grid_res", ctypes.c_float),
+                ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
+                ("min_range", ctypes.c_float), ("max_range", ctypes.c_float)]
+
+
 class Camera(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
@@ -116,6 +124,13 @@ SIGNATURES = {
                            _PF]),
     "nm_frame_from_pose": (_I, [_PF, _PF, _PF, _F, _F, _F, _F, _I,
                                 ctypes.POINTER(Frame)]),
+    "nm_simgrid_from_leaves": (_I, [_I, _PI32, _PF, ctypes.c_int64, _F,
+                                    ctypes.POINTER(_P)]),
+    "nm_simgrid_from_map": (_I, [_P, _F, ctypes.POINTER(_P)]),
+    "nm_simgrid_destroy": (_I, [_P]),
+    "nm_simgrid_info": (_I, [_P, _PI64, _PI32, _PI32]),
+    "nm_sim_cast": (_I, [_P, ctypes.POINTER(SimSensor), _I, _P, _P, _P,
+                         ctypes.POINTER(_I), _P]),
 }
 
 _lib = None

@@ -42,3 +42,24 @@ def cfg2_frames(nm, n_frames=100, seed=2, first=0):
 def frame(nm, pos, quat, res, rng, rounding, w=0, h=0, filt=0):
     return nm.frame_from_pose(pos, quat, nm.DEFAULT_F, res, rng, HIT, MISS,
                               rounding, w, h, filt)
+
+
+def cfg3_poses(nm, n_poses=64, seed=3):
+    """Sensor-simulation poses in the config-1 room (same world seed)."""
+    scene = nm.Scene(1, (20.0, 20.0, 5.0), 40, 0.3, 2.0)
+    out = []
+    for k in range(n_poses):
+        out.append(scene.sample_pose(seed * 1000 + k, 15.0, 0.3))
+    return out
+
+
+def cfg1_map_leaves(nm, oracle_map=None):
+    """(origins, values) of the config-1 final map computed on the device."""
+    _, cam, frames = cfg1(nm)
+    pos, quat, cloud = frames[0]
+    f = frame(nm, pos, quat, 0.1, 5.0, nm.ROUND_DOWN, cam.width, cam.height)
+    ctx = nm.Context(cam.width * cam.height, f.max_time)
+    m = nm.Map(0.1, *CLAMP)
+    ctx.submit(m, f, cloud)
+    o, v, _ = m.download()
+    return o, v

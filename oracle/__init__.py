@@ -40,6 +40,11 @@ SIGNATURES = {
                           ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "or_fold": (_F, [_F, _F, ctypes.c_uint32, _I]),
     "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _PF, _I]),
+    "or_simgrid_build": (_P, [_PI32, _PF, ctypes.c_int64, _F]),
+    "or_simgrid_destroy": (None, [_P]),
+    "or_simgrid_active_count": (ctypes.c_int64, [_P]),
+    "or_sim_cast": (_I, [_P, _PF, _PF, _F, _I, _I, _F, _F, _F, _F, _I, _PI32,
+                         _PF, _PF]),
 }
 
 _lib = None
@@ -190,3 +195,38 @@ def voxel_filter(frame, cloud, max_rays=None):
                               frame.grid_res, frame.max_time,
                               _p(out, ctypes.c_float), max_rays)
     return out[:min(n, max_rays)]
+
+
+class SimGrid:
+    """Occupancy grid of the sensor-simulation cast (value > threshold)."""
+
+    def __init__(self, origins, values, threshold=0.0):
+        o = np.ascontiguousarray(origins, np.int32).reshape(-1, 3)
+        v = np.ascontiguousarray(values, np.float32).reshape(-1, 512)
+        self.h = lib().or_simgrid_build(_p(o, ctypes.c_int32),
+                                        _p(v, ctypes.c_float), o.shape[0],
+                                        threshold)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_simgrid_destroy(self.h)
+            self.h = None
+
+    def active_count(self):
+        return int(lib().or_simgrid_active_count(self.h))
+
+    def cast(self, pos, R, res, width, height, hfov, vfov, min_range,
+             max_range, hierarchical=True):
+        """(ijk [P,3], dist [P], xyz [P,3], returns) of one frustum."""
+        P = width * height
+        ijk = np.zeros((P, 3), np.int32)
+        dist = np.zeros(P, np.float32)
+        xyz = np.zeros((P, 3), np.float32)
+        p = np.ascontiguousarray(pos, np.float32)
+        Rm = np.ascontiguousarray(R, np.float32).reshape(9)
+        n = lib().or_sim_cast(self.h, _p(p, ctypes.c_float),
+                              _p(Rm, ctypes.c_float), res, width, height,
+                              hfov, vfov, min_range, max_range,
+                              int(hierarchical), _p(ijk, ctypes.c_int32),
+                              _p(dist, ctypes.c_float), _p(xyz, ctypes.c_float))
+        return ijk, dist, xyz, n

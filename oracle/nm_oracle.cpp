@@ -13,6 +13,8 @@
 #include "nm_oracle.h"
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cfenv>
 #include <cfloat>
 #include <cmath>
@@ -507,6 +509,257 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
     ++n;
   }
   return n;
+}
+
+// ---------------------------------------------------------------------
+// Sensor-simulation cast: restatement of viewSimKernel + getActiveCrossing
+// (generateCloud.cu:20-82) over a NanoVDB-style 5-4-3 tree, with the
+// frustum ray of gpu::Sensor::updateFrustum/getRay (Sensor.h:156-227)
+// taking an independent horizontal FOV (the reference derives it from the
+// pixel aspect, FrustumData.h:24, Sensor.h:160-161).  FP32, round to
+// nearest (the reference runs it on the GPU).
+// ---------------------------------------------------------------------
+}  // extern "C"
+
+struct or_simgrid {
+  std::unordered_map<uint64_t, std::array<uint64_t, 8>> leaves;  // mask
+  std::unordered_map<uint64_t, int> lowers;  // origin key -> 1
+  std::unordered_map<uint64_t, int> uppers;
+  int bmin[3], bmax[3];
+  int64_t active = 0;
+};
+
+namespace {
+
+inline uint64_t node_key(int x, int y, int z, int mask) {
+  const uint64_t m = (1ull << 21) - 1;
+  return ((uint64_t)(((x & ~mask) >> 3) & m) << 42) |
+         ((uint64_t)(((y & ~mask) >> 3) & m) << 21) |
+         (uint64_t)(((z & ~mask) >> 3) & m);
+}
+
+// ReadAccessor::getDim: 1 inside a leaf, 8 in a lower node without the
+// leaf, 128 in an upper node without the lower node, 4096 outside.
+inline int sim_get_dim(const or_simgrid* g, int x, int y, int z) {
+  if (!g->uppers.count(node_key(x, y, z, 4095))) return 4096;
+  if (!g->lowers.count(node_key(x, y, z, 127))) return 128;
+  if (!g->leaves.count(node_key(x, y, z, 7))) return 8;
+  return 1;
+}
+
+inline bool sim_active(const or_simgrid* g, int x, int y, int z) {
+  auto it = g->leaves.find(node_key(x, y, z, 7));
+  if (it == g->leaves.end()) return false;
+  const int li = local_index(x, y, z);
+  return (it->second[li >> 6] >> (li & 63)) & 1;
+}
+
+inline int min_index_f(const float n[3]) {
+  if (n[0] < n[1] && n[0] < n[2]) return 0;
+  if (n[1] < n[2]) return 1;
+  return 2;
+}
+
+struct FRay {
+  float eye[3], dir[3], inv[3];
+  float t0, t1;
+  void at(float t, float p[3]) const {
+    for (int a = 0; a < 3; ++a) p[a] = eye[a] + dir[a] * t;
+  }
+};
+
+// nanovdb::HDDA over FP32 rays.
+struct HDda {
+  int dim;
+  float t0, t1;
+  int voxel[3], step[3];
+  float next[3], delta[3];
+  void init(const FRay& r, float start, float maxt, int d) {
+    dim = d;
+    t0 = start;
+    t1 = maxt;
+    float pos[3];
+    r.at(t0, pos);
+    for (int a = 0; a < 3; ++a) {
+      voxel[a] = ((int)std::floor(pos[a])) & ~(d - 1);
+      if (r.dir[a] == 0.0f) {
+        next[a] = FLT_MAX;
+        step[a] = 0;
+        delta[a] = 0.0f;
+      } else if (r.inv[a] > 0) {
+        step[a] = 1;
+        next[a] = t0 + ((float)(voxel[a] + d) - pos[a]) * r.inv[a];
+        delta[a] = r.inv[a];
+      } else {
+        step[a] = -1;
+        next[a] = t0 + ((float)voxel[a] - pos[a]) * r.inv[a];
+        delta[a] = -r.inv[a];
+      }
+    }
+  }
+  void update(const FRay& r, int d) {
+    if (dim == d) return;
+    dim = d;
+    float pos[3];
+    r.at(t0, pos);
+    for (int a = 0; a < 3; ++a) voxel[a] = ((int)std::floor(pos[a])) & ~(d - 1);
+    for (int a = 0; a < 3; ++a) {
+      if (step[a] == 0) continue;
+      next[a] = t0 + ((float)voxel[a] - pos[a]) * r.inv[a];
+      if (step[a] > 0) next[a] += (float)d * r.inv[a];
+    }
+  }
+  bool step_() {
+    const int a = min_index_f(next);
+    t0 = next[a];
+    next[a] += (float)dim * delta[a];
+    voxel[a] += dim * step[a];
+    return t0 <= t1;
+  }
+};
+
+// Ray::clip(CoordBBox): slabs of [min, max + 1).
+bool clip_bbox(FRay& r, const int bmin[3], const int bmax[3]) {
+  float t0 = r.t0, t1 = r.t1;
+  for (int a = 0; a < 3; ++a) {
+    float lo = (float)bmin[a], hi = (float)(bmax[a] + 1);
+    if (lo >= hi) return false;
+    lo = (lo - r.eye[a]) * r.inv[a];
+    hi = (hi - r.eye[a]) * r.inv[a];
+    if (lo > hi) std::swap(lo, hi);
+    if (lo > t0) t0 = lo;
+    if (hi < t1) t1 = hi;
+    if (t0 > t1) return false;
+  }
+  r.t0 = t0;
+  r.t1 = t1;
+  return true;
+}
+
+bool active_crossing(const or_simgrid* g, FRay r, bool hier, int ijk[3],
+                     float* t) {
+  if (g->active == 0) return false;
+  if (!clip_bbox(r, g->bmin, g->bmax) || r.t1 > 1e20f) return false;
+  HDda h;
+  h.init(r, r.t0, r.t1, 1);
+  while (h.step_()) {
+    if (hier) {
+      const int d = sim_get_dim(g, h.voxel[0], h.voxel[1], h.voxel[2]);
+      if (h.dim != d) h.update(r, d);
+      if (h.dim > 1) continue;
+    }
+    if (sim_active(g, h.voxel[0], h.voxel[1], h.voxel[2])) {
+      for (int a = 0; a < 3; ++a) ijk[a] = h.voxel[a];
+      *t = h.t0;
+      return true;
+    }
+  }
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+or_simgrid* or_simgrid_build(const int32_t* origins, const float* values,
+                             int64_t n_leaves, float threshold) {
+  or_simgrid* g = new or_simgrid();
+  for (int a = 0; a < 3; ++a) {
+    g->bmin[a] = INT32_MAX;
+    g->bmax[a] = INT32_MIN;
+  }
+  for (int64_t l = 0; l < n_leaves; ++l) {
+    std::array<uint64_t, 8> mask{};
+    bool any = false;
+    const int* o = origins + 3 * l;
+    for (int li = 0; li < 512; ++li) {
+      if (values[512 * l + li] > threshold) {
+        mask[li >> 6] |= 1ull << (li & 63);
+        any = true;
+        const int v[3] = {o[0] + (li >> 6), o[1] + ((li >> 3) & 7), o[2] + (li & 7)};
+        for (int a = 0; a < 3; ++a) {
+          g->bmin[a] = std::min(g->bmin[a], v[a]);
+          g->bmax[a] = std::max(g->bmax[a], v[a]);
+        }
+        g->active += 1;
+      }
+    }
+    if (!any) continue;
+    g->leaves[node_key(o[0], o[1], o[2], 7)] = mask;
+    g->lowers[node_key(o[0], o[1], o[2], 127)] = 1;
+    g->uppers[node_key(o[0], o[1], o[2], 4095)] = 1;
+  }
+  return g;
+}
+
+void or_simgrid_destroy(or_simgrid* g) { delete g; }
+
+int64_t or_simgrid_active_count(const or_simgrid* g) { return g->active; }
+
+int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
+                float res, int width, int height, float hfov_deg,
+                float vfov_deg, float min_range, float max_range,
+                int hierarchical, int32_t* ijk, float* dist, float* xyz) {
+  RoundGuard guard(0);
+  // Sensor::updateFrustum (Sensor.h:156-171)
+  const float hh = (float)std::tan(vfov_deg * 3.14159265358979323846 / 360);
+  const float hw = (float)std::tan(hfov_deg * 3.14159265358979323846 / 360);
+  const float s0 = (float)(1.0 / width), s1 = (float)(1.0 / height);
+  float T[3][3], I[3][3];  // T[c][i]: column c, component i
+  for (int i = 0; i < 3; ++i) {
+    const float r0 = R[3 * i + 0], r1 = R[3 * i + 1], r2 = R[3 * i + 2];
+    T[2][i] = hw * r0 + hh * r1 - r2;
+    T[0][i] = r0 * (2 * hw);
+    T[1][i] = r1 * (2 * hh);
+  }
+  const float e0[3] = {1, 0, 0}, e1[3] = {0, 1, 0}, e2[3] = {0, 0, 1};
+  for (int i = 0; i < 3; ++i) {
+    I[2][i] = hw * e0[i] + hh * e1[i] - e2[i];
+    I[0][i] = e0[i] * 2 * hw;
+    I[1][i] = e1[i] * 2 * hh;
+  }
+  int returns = 0;
+  for (int h = 0; h < height; ++h) {
+    for (int w = 0; w < width; ++w) {
+      const size_t pix = (size_t)h * width + w;
+      // Sensor::getRay, frustum branch (Sensor.h:215-227)
+      const float u = (float)w * s0, v = (float)h * s1;
+      float md[3], dd[3];
+      for (int i = 0; i < 3; ++i) {
+        md[i] = u * T[0][i] + v * T[1][i] - T[2][i];
+        dd[i] = u * I[0][i] + v * I[1][i] - I[2][i];
+      }
+      const float norm = std::sqrt(md[0] * md[0] + md[1] * md[1] + md[2] * md[2]);
+      const float dn = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+      for (int i = 0; i < 3; ++i) dd[i] = dd[i] / dn;
+      for (int i = 0; i < 3; ++i) md[i] = md[i] / norm;
+      const float min_t = (min_range / res) * norm;
+      const float max_t = (max_range / res) * norm;
+      FRay r;
+      for (int a = 0; a < 3; ++a) {
+        r.eye[a] = pos[a] / res;
+        r.dir[a] = md[a];
+        r.inv[a] = 1.0f / md[a];
+      }
+      r.t0 = 1.0e-5f;
+      r.t1 = FLT_MAX;
+      int vox[3];
+      float t = 0;
+      const bool hit = active_crossing(g, r, hierarchical != 0, vox, &t);
+      if (hit && t >= min_t && t <= max_t) {
+        const float d = t * res;
+        if (ijk) for (int a = 0; a < 3; ++a) ijk[3 * pix + a] = vox[a];
+        if (dist) dist[pix] = d;
+        if (xyz) for (int a = 0; a < 3; ++a) xyz[3 * pix + a] = dd[a] * d;
+        ++returns;
+      } else {
+        if (ijk) for (int a = 0; a < 3; ++a) ijk[3 * pix + a] = INT32_MIN;
+        if (dist) dist[pix] = -1.0f;
+        if (xyz) for (int a = 0; a < 3; ++a) xyz[3 * pix + a] = NAN;
+      }
+    }
+  }
+  return returns;
 }
 
 }  // extern "C"

@@ -81,6 +81,27 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     const float R[9], const double eye[3], float res,
                     float max_time, float* rays4, int max_rays);
 
+/* ---- sensor-simulation cast (generateCloud.cu:20-82, Sensor.h:156-227) - */
+typedef struct or_simgrid or_simgrid;
+/* Occupancy grid: voxels with value > threshold of the given leaves
+ * (origins: 3 ints/leaf, values: 512 floats/leaf).  Topology: a leaf exists
+ * iff it holds an active voxel; internal nodes of NanoVDB's 5-4-3 tree
+ * (16^3 leaves per lower node, 32^3 lower nodes per upper node) exist iff
+ * they hold a leaf. */
+or_simgrid* or_simgrid_build(const int32_t* origins, const float* values,
+                             int64_t n_leaves, float threshold);
+void or_simgrid_destroy(or_simgrid* g);
+int64_t or_simgrid_active_count(const or_simgrid* g);
+/* Cast a W x H frustum.  Per pixel (row-major): first active voxel ijk
+ * (INT32_MIN if none in range), hit distance t*res in metres (-1 if none),
+ * and the simulated sensor-frame point (NaN if no return).  hierarchical:
+ * 1 = NanoVDB HDDA with node dims 1/8/128/4096, 0 = plain dim-1 DDA.
+ * Returns the number of returns. */
+int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
+                float res, int width, int height, float hfov_deg,
+                float vfov_deg, float min_range, float max_range,
+                int hierarchical, int32_t* ijk, float* dist, float* xyz);
+
 #ifdef __cplusplus
 }
 #endif
