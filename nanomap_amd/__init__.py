@@ -82,7 +82,7 @@ class Context:
             st = step or cloud.stride(0) * cloud.element_size()
             return ctypes.c_void_p(cloud.data_ptr()), n, st, int(cloud.is_cuda)
         a = np.ascontiguousarray(cloud, dtype=np.float32)
-        a = a.reshape(a.shape[0], -1)
+        a = a.reshape(a.shape[0], -1) if a.size else a.reshape(0, 3)
         return a.ctypes.data_as(ctypes.c_void_p), a.shape[0], a.shape[1] * 4, 0
 
     def cast(self, frame, cloud, step=0, stream=None):

@@ -137,9 +137,18 @@ _lib = None
 
 
 def lib():
-    """Load libnanomap_amd.so (raises if it has not been built)."""
+    """Load libnanomap_amd.so (raises if it has not been built).
+
+    torch, when installed, is imported first: its bundled HIP runtime then
+    satisfies this library's libamdhip64.so.7 dependency, so one process
+    never holds two HIP runtimes (torch's libraries ask for the unversioned
+    libamdhip64.so and would otherwise load a second copy)."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise NanoMapError(NM_ENODEV,
                                "%s is not built: run __graft_entry__.build()"

@@ -220,6 +220,14 @@ __global__ __launch_bounds__(256) void k_sim_cast(
       }
       Acc acc;
       float tc = t0;
+      // Livelock breaker (same rule as the CPU restatement): after kStall
+      // steps at one t, coarsening is suppressed until t advances; steps
+      // are capped so every lane exits.
+      const int kStall = 8;
+      int stall = 0;
+      float tprev = -1.0f;
+      const double cap = 6.0 * ((double)t1 - (double)t0 + 16.0) + 4096.0;
+      double steps = 0;
       for (;;) {
         // HDDA::step()
         const int ax = fmin_index(next[0], next[1], next[2]);
@@ -227,9 +235,12 @@ __global__ __launch_bounds__(256) void k_sim_cast(
         next[ax] += (float)dim * delta[ax];
         vx[ax] += dim * st[ax];
         if (!(tc <= t1)) break;
+        if (++steps > cap) break;
+        stall = (tc == tprev) ? stall + 1 : 0;
+        tprev = tc;
         int leaf = -1;
         const int nd = get_dim(g, acc, vx[0], vx[1], vx[2], &leaf);
-        if (nd != dim) {
+        if (nd != dim && !(stall >= kStall && nd > dim)) {
           // HDDA::update(ray, nd): re-anchor at ray(t)
           dim = nd;
 #pragma unroll

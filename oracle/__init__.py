@@ -91,7 +91,7 @@ class Temp:
 
     def cast_cloud(self, frame, cloud):
         a = np.ascontiguousarray(cloud, dtype=np.float32)
-        a = a.reshape(a.shape[0], -1)
+        a = a.reshape(a.shape[0], -1) if a.size else a.reshape(0, 3)
         R, eye = _frame_args(frame)
         return lib().or_cast_cloud(self.h, a.ctypes.data_as(_P), a.shape[0],
                                    a.shape[1] * 4, _p(R, ctypes.c_float),
@@ -120,7 +120,9 @@ class Temp:
         lib().or_temp_export(self.h, _p(ijk, ctypes.c_int32),
                              _p(miss, ctypes.c_uint32), _p(hit, ctypes.c_uint32),
                              _p(val, ctypes.c_float))
-        return ijk[:n], miss[:n], hit[:n], val[:n]
+        ijk, miss, hit, val = ijk[:n], miss[:n], hit[:n], val[:n]
+        order = np.lexsort((ijk[:, 2], ijk[:, 1], ijk[:, 0]))
+        return ijk[order], miss[order], hit[order], val[order]
 
 
 class Map:
@@ -186,7 +188,7 @@ def fold(v, a, k, rounding):
 
 def voxel_filter(frame, cloud, max_rays=None):
     a = np.ascontiguousarray(cloud, dtype=np.float32)
-    a = a.reshape(a.shape[0], -1)
+    a = a.reshape(a.shape[0], -1) if a.size else a.reshape(0, 3)
     R, eye = _frame_args(frame)
     max_rays = max_rays or a.shape[0]
     out = np.zeros((max(max_rays, 1), 4), np.float32)

@@ -642,10 +642,24 @@ bool active_crossing(const or_simgrid* g, FRay r, bool hier, int ijk[3],
   if (!clip_bbox(r, g->bmin, g->bmax) || r.t1 > 1e20f) return false;
   HDda h;
   h.init(r, r.t0, r.t1, 1);
+  // Livelock breaker: a ray through an edge where two node boundaries
+  // meet can re-anchor back and forth forever with t frozen (refine to a
+  // leaf, step, coarsen back to the same anchor).  After kStall steps at
+  // one t, coarsening is suppressed until t advances.  Rays that never
+  // stall are unaffected.  Steps are also capped (never reached by rays
+  // that make progress).
+  const int kStall = 8;
+  int stall = 0;
+  float tprev = -1.0f;
+  const double cap = 6.0 * ((double)r.t1 - (double)r.t0 + 16.0) + 4096.0;
+  double steps = 0;
   while (h.step_()) {
+    if (++steps > cap) return false;
     if (hier) {
+      stall = (h.t0 == tprev) ? stall + 1 : 0;
+      tprev = h.t0;
       const int d = sim_get_dim(g, h.voxel[0], h.voxel[1], h.voxel[2]);
-      if (h.dim != d) h.update(r, d);
+      if (h.dim != d && !(stall >= kStall && d > h.dim)) h.update(r, d);
       if (h.dim > 1) continue;
     }
     if (sim_active(g, h.voxel[0], h.voxel[1], h.voxel[2])) {

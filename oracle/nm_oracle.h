@@ -42,7 +42,8 @@ int or_cast_rays(or_temp* t, const float* rays4, int n_rays,
                  float logodds_miss, int rounding);
 
 int64_t or_temp_size(const or_temp* t);
-/* Export touched voxels sorted by (i, j, k). Any pointer may be NULL. */
+/* Export touched voxels, leaf by leaf (leaves sorted by origin, voxels in
+ * z + 8y + 64x order).  Any pointer may be NULL. */
 void or_temp_export(const or_temp* t, int32_t* ijk, uint32_t* miss,
                     uint32_t* hit, float* value);
 uint64_t or_temp_visits(const or_temp* t);

@@ -78,9 +78,11 @@ def test_sim_on_dense_map_and_feedback(gpu):
     osim = oracle.SimGrid(o, v, 0.0)
     poses = [(p, q) for p, q, _ in frames]
     sens = _sensors(nm, poses, 0.05, 320, 240, 87.0, 58.0, 0.1, 10.0)
+    print("dense sim cast: 3 x 320x240, active", gsim.info()["active"], flush=True)
     ijk, dist, xyz, n = gsim.cast([s for _, _, s in sens])
     xyz = xyz.cpu().numpy()
     ijk = ijk.cpu().numpy()
+    print("dense sim cast done, returns", n, flush=True)
     P = 320 * 240
     ctx = nm.Context(P, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)

@@ -109,3 +109,24 @@ def test_empty_grid_returns_nothing(oracle_lib):
     ijk, d, xyz, n = g.cast([0, 0, 0], np.eye(3, dtype=np.float32), 0.1, 16, 16,
                             60.0, 45.0, 0.1, 10.0)
     assert n == 0 and np.all(d == -1) and np.all(np.isnan(xyz))
+
+
+def test_dense_grid_terminates(oracle_lib):
+    """Regression: on a dense 0.05 m grid some rays pass exactly through an
+    edge where two node boundaries meet; re-anchoring at ray(t) then
+    alternates refine/coarsen with t frozen.  The restated HDDA suppresses
+    coarsening after 8 steps at one t, so every ray terminates."""
+    import nanomap_amd as nm
+    from nanomap_amd import workloads as W
+    _, cam, frames = W.cfg2_frames(nm, n_frames=3)
+    om = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN)
+        oracle.process_frame(om, f, cloud)
+    o, v, _ = om.export()
+    g = oracle.SimGrid(o, v, 0.0)
+    for pos, quat, _ in frames:
+        fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 10.0, 0.85, -0.4, 0)
+        R = np.array(list(fr.rotation), np.float32)
+        _, d, _, n = g.cast(pos, R, 0.05, 320, 240, 87.0, 58.0, 0.1, 10.0)
+        assert n > 40000
