@@ -38,6 +38,7 @@ struct nm_ctx {
   int sorted_buf = 0;
   hipStream_t stream = nullptr;
   nm_map* last_map = nullptr;
+  bool last_update_valid = false;  // part_update holds the last frame's sums
   // stage timing
   bool timing = false;
   struct Mark {
@@ -157,10 +158,11 @@ void free_ctx(nm_ctx* c) {
   }
   for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
   nm::CtxBuffers& b = c->b;
-  void* ptrs[] = {b.leaf_slot,  b.slot_leaf,  b.tiles,      b.hit_mask,
+  void* ptrs[] = {b.leaf_slot,  b.slot_leaf, b.map_idx,  b.tiles,      b.hit_mask, b.hit_start,
                   b.ray_hit,    b.keys[0],    b.keys[1],    b.vals[0],
                   b.vals[1],    b.gap,        b.radix_hist, b.scan_blocks,
-                  b.counters,   b.cloud,      b.filter_rays, b.filter_tmp,
+                  b.counters,   b.part_discover, b.part_update,
+                  b.cloud,      b.filter_rays, b.filter_tmp,
                   b.flags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -216,17 +218,23 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   };
   A(dalloc(&b.leaf_slot, (size_t)c->cube_cap));
   A(dalloc(&b.slot_leaf, (size_t)c->tile_cap));
+  A(dalloc(&b.map_idx, (size_t)c->tile_cap));
   A(dalloc(&b.tiles, (size_t)c->tile_cap * 1024));
   A(dalloc(&b.hit_mask, (size_t)c->tile_cap * nm::kMaskWords));
+  A(dalloc(&b.hit_start, (size_t)c->tile_cap * 512));
   A(dalloc(&b.ray_hit, P));
   A(dalloc(&b.keys[0], P));
   A(dalloc(&b.keys[1], P));
   A(dalloc(&b.vals[0], P));
   A(dalloc(&b.vals[1], P));
   A(dalloc(&b.gap, P));
-  A(dalloc(&b.radix_hist, (size_t)256 * (nm::radix_max_blocks((int)P) + 1)));
+  A(dalloc(&b.radix_hist, (size_t)nm::radix_hist_words((int)P)));
   A(dalloc(&b.scan_blocks, scan_blocks));
   A(dalloc(&b.counters, 1));
+  // discover grid: at most ceil(W/16)*ceil(H/16) <= P/256 + (W+H)/16 + 1
+  // <= P/16 + P/256 + 2 blocks (W + H <= W*H + 1), or P/256 + 1
+  A(dalloc(&b.part_discover, (P / 16 + P / 256 + 64) * 4));
+  A(dalloc(&b.part_update, (size_t)nm::kUpdateGrid * 4));
   A(dalloc(&b.cloud, c->staging_bytes));
   A(dalloc(&b.filter_rays, P * 4));
   A(dalloc(&b.filter_tmp, P * 4));
@@ -413,6 +421,12 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.n = n_points;
   p.step = step;
   p.width = frame->width;
+  p.height = frame->height;
+  p.tiled = (frame->width > 0 && frame->height > 0 &&
+             (int64_t)frame->width * frame->height == n_points &&
+             frame->filter == NM_FILTER_NONE)
+                ? 1
+                : 0;
   p.use_rays = 0;
   p.rays4 = nullptr;
   p.n_dev = nullptr;
@@ -435,7 +449,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   }
   {
     StageTimer t(ctx, NM_STAGE_COMPACT, s);
-    NM_HIP(nm::launch_compact_leaves(ctx->b, vol, ctx->tile_cap, s));
+    NM_HIP(nm::launch_compact_leaves(p, ctx->b, vol, ctx->tile_cap, s));
   }
   int sorted = 0;
   {
@@ -451,6 +465,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     NM_HIP(nm::launch_accumulate(frame->rounding, p, ctx->b, sorted, s));
   }
   ctx->pending = true;
+  ctx->last_update_valid = false;
   ctx->frame = *frame;
   ctx->cube = cube;
   ctx->n = n_points;
@@ -473,6 +488,7 @@ int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
                              ctx->sorted_buf, ctx->tile_cap, s));
   }
   ctx->pending = false;
+  ctx->last_update_valid = true;
   ctx->last_map = map;
   ctx->stream = s;
   map->stream = s;
@@ -495,14 +511,22 @@ int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out) {
                         ctx->stream));
   NM_HIP(hipStreamSynchronize(ctx->stream));
   const nm::FrameCounters& c = *ctx->host_counters;
+  uint64_t upd[3] = {0, 0, 0};
+  if (ctx->last_update_valid) {
+    std::vector<uint64_t> part((size_t)nm::kUpdateGrid * 4);
+    NM_HIP(hipMemcpy(part.data(), ctx->b.part_update,
+                     part.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (int b = 0; b < nm::kUpdateGrid; ++b)
+      for (int q = 0; q < 3; ++q) upd[q] += part[(size_t)b * 4 + q];
+  }
   memset(out, 0, sizeof(*out));
   out->points = ctx->n;
   out->rays = (int64_t)c.rays;
   out->visits = (int64_t)c.visits;
-  out->voxels = (int64_t)c.voxels;
+  out->voxels = (int64_t)upd[0];
   out->leaves = c.n_leaves;
-  out->hit_voxels = (int64_t)c.hit_voxels;
-  out->mixed_voxels = (int64_t)c.mixed_voxels;
+  out->hit_voxels = (int64_t)upd[1];
+  out->mixed_voxels = (int64_t)upd[2];
   out->errors = c.errors;
   out->overflow = c.overflow;
   if (ctx->last_map) {

@@ -76,20 +76,61 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   return v;
 }
 
+// Point handled by thread t of block b.  Image clouds are walked in 16x16
+// pixel tiles (one 8x8 sub-tile per wavefront) so the rays of a wavefront
+// stay close together: less divergence in the DDA, more shared voxels for
+// the LDS aggregation.  Other clouds are walked linearly.
+__device__ __forceinline__ int point_of(const CastParams& p, int b, int t) {
+  if (p.tiled) {
+    const int tiles_x = (p.width + 15) >> 4;
+    const int sub = t >> 6, lane = t & 63;
+    const int x = (b % tiles_x) * 16 + (sub & 1) * 8 + (lane & 7);
+    const int y = (b / tiles_x) * 16 + (sub >> 1) * 8 + (lane >> 3);
+    if (x >= p.width || y >= p.height) return -1;
+    return y * p.width + x;
+  }
+  const int i = b * kCastThreads + t;
+  return i < p.n ? i : -1;
+}
+
+__host__ __device__ inline int cast_grid(const CastParams& p) {
+  if (p.tiled) return ((p.width + 15) >> 4) * ((p.height + 15) >> 4);
+  return (p.n + kCastThreads - 1) / kCastThreads;
+}
+
 __global__ void k_reset_counters(FrameCounters* c) {
   if (threadIdx.x == 0) *c = FrameCounters{};
+}
+
+// Per-block partial sums of the frame counters, written with plain stores
+// and reduced by one workgroup later: a global atomic per wave on a handful
+// of addresses serializes at the memory side and cost more than the
+// traversal itself.
+__device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
+                                               const uint64_t* v, int n) {
+  __shared__ uint64_t s_part[4][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < n; ++q) {
+    const uint64_t w = wave_sum(v[q]);
+    if (lane == 0) s_part[wave][q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < n)
+    out[(size_t)blockIdx.x * 4 + threadIdx.x] =
+        s_part[0][threadIdx.x] + s_part[1][threadIdx.x] +
+        s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
 }
 
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
     CastParams p, int32_t* __restrict__ leaf_slot,
-    uint32_t* __restrict__ ray_hit, FrameCounters* __restrict__ cnt) {
-  const int i = blockIdx.x * kCastThreads + threadIdx.x;
+    uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part) {
+  const int i = point_of(p, blockIdx.x, threadIdx.x);
   uint32_t hit_cell = kInvalid;
   uint32_t visits = 0;
   int valid = 0, errors = 0;
   RayDir r;
-  if (i < p.n && ray_of<M>(p, i, r)) {
+  if (i >= 0 && ray_of<M>(p, i, r)) {
     valid = 1;
     Dda s;
     dda_init<M>(p.eye, r, s);
@@ -118,17 +159,10 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
       if (ce >= 0) hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(px, py, pz);
     }
   }
-  if (i < p.n) ray_hit[i] = hit_cell;
-  const uint64_t sv = wave_sum(visits);
-  const uint64_t sr = wave_sum((uint64_t)valid);
-  const uint64_t sh = wave_sum((uint64_t)(hit_cell != kInvalid));
-  const uint64_t se = wave_sum((uint64_t)errors);
-  if ((threadIdx.x & 63) == 0) {
-    if (sv) atomicAdd((unsigned long long*)&cnt->visits, (unsigned long long)sv);
-    if (sr) atomicAdd((unsigned long long*)&cnt->rays, (unsigned long long)sr);
-    if (sh) atomicAdd(&cnt->n_hit_rays, (int)sh);
-    if (se) atomicAdd(&cnt->errors, (int)se);
-  }
+  if (i >= 0) ray_hit[i] = hit_cell;
+  const uint64_t v[4] = {visits, (uint64_t)valid,
+                         (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
+  block_partials(part, v, 4);
 }
 
 // ---- leaf compaction: exclusive scan over the cube table --------------
@@ -148,12 +182,33 @@ __global__ __launch_bounds__(256) void k_scan_count(
   if (threadIdx.x == 0) block_counts[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-// One workgroup: exclusive scan of block_counts; total -> n_leaves.
+// One workgroup: exclusive scan of block_counts; total -> n_leaves.  Also
+// reduces the discover partials (visits, rays, hit rays, errors).
 __global__ __launch_bounds__(1024) void k_scan_blocks(
     int32_t* __restrict__ block_counts, int nblocks,
+    const uint64_t* __restrict__ part, int npart,
     FrameCounters* __restrict__ cnt) {
   __shared__ int wsum[16];
+  __shared__ unsigned long long psum[4];
   const int tid = threadIdx.x;
+  if (tid < 4) psum[tid] = 0;
+  __syncthreads();
+  {
+    uint64_t acc[4] = {0, 0, 0, 0};
+    for (int b = tid; b < npart; b += 1024)
+      for (int q = 0; q < 4; ++q) acc[q] += part[(size_t)b * 4 + q];
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t w = wave_sum(acc[q]);
+      if ((tid & 63) == 0 && w) atomicAdd(&psum[q], (unsigned long long)w);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    cnt->visits += psum[0];
+    cnt->rays += psum[1];
+    cnt->n_hit_rays += (int32_t)psum[2];
+    cnt->errors += (int32_t)psum[3];
+  }
   const int per = (nblocks + 1023) / 1024;
   const int b0 = tid * per, b1 = min(nblocks, b0 + per);
   int local = 0;
@@ -234,26 +289,10 @@ __global__ __launch_bounds__(256) void k_hit_keys(
   vals[i] = (uint32_t)i;
 }
 
-// First index e in [lo, hi) with (keys[e], vals[e]) >= (key, val).
-__device__ __forceinline__ int lower_bound_pair(const uint32_t* keys,
-                                                const uint32_t* vals, int lo,
-                                                int hi, uint32_t key,
-                                                uint32_t val) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const uint32_t k = keys[mid];
-    if (k < key || (k == key && vals[mid] < val))
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 __global__ __launch_bounds__(256) void k_hit_segments(
     const uint32_t* __restrict__ keys, const FrameCounters* __restrict__ cnt,
     uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
-    uint32_t* __restrict__ gap) {
+    uint32_t* __restrict__ hit_start, uint32_t* __restrict__ gap) {
   const int nh = cnt->n_hit_rays;
   for (int e = blockIdx.x * 256 + threadIdx.x; e < nh;
        e += gridDim.x * 256) {
@@ -271,36 +310,93 @@ __global__ __launch_bounds__(256) void k_hit_segments(
         hi = mid;
     }
     tiles[2 * (size_t)key + 1] = (uint32_t)(lo - e);
+    hit_start[key] = (uint32_t)e;
     const uint32_t slot = key >> 9, li = key & 511;
     atomicOr(&hit_mask[slot * kMaskWords + (li >> 5)], 1u << (li & 31));
   }
+}
+
+// LDS aggregation of miss counts: an open-addressing table of (cell,
+// count) per workgroup.  A 16x16 pixel tile touches a few thousand distinct
+// voxels for tens of thousands of visits, so the table turns almost every
+// visit into an LDS atomic and leaves one global atomic per distinct voxel
+// per workgroup, at flush.  A visit that finds no free slot within kProbe
+// probes falls back to a global atomic.
+constexpr int kLdsSlots = 4096;
+constexpr int kProbe = 16;
+
+__device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
+                                          uint32_t* tiles, uint32_t cell,
+                                          uint32_t inc) {
+  uint32_t h = (cell * 2654435761u) >> 20;  // 12-bit multiplicative hash
+#pragma unroll 1
+  for (int q = 0; q < kProbe; ++q) {
+    const uint32_t k = s_key[h];
+    if (k == cell) {
+      atomicAdd(&s_cnt[h], inc);
+      return;
+    }
+    if (k == kInvalid) {
+      const uint32_t old = atomicCAS(&s_key[h], kInvalid, cell);
+      if (old == kInvalid || old == cell) {
+        atomicAdd(&s_cnt[h], inc);
+        return;
+      }
+    }
+    h = (h + 1) & (kLdsSlots - 1);
+  }
+  atomicAdd(&tiles[2 * (size_t)cell], inc);
+}
+
+// lds_count for the active lanes; when they all name the same voxel (the
+// voxels next to the eye) one lane adds the lane count.
+__device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
+                                               uint32_t* s_cnt,
+                                               uint32_t* tiles,
+                                               uint32_t cell) {
+  const uint64_t act = __ballot(1);
+  const uint32_t first = __builtin_amdgcn_readfirstlane(cell);
+  const uint64_t same = __ballot(cell == first);
+  if (same == act) {
+    const int lane = threadIdx.x & 63;
+    if (lane == (int)__builtin_ctzll(act))
+      lds_count(s_key, s_cnt, tiles, cell, (uint32_t)__popcll(act));
+  } else {
+    lds_count(s_key, s_cnt, tiles, cell, 1u);
+  }
+}
+
+__device__ __forceinline__ int slot_of(const Cube& cube,
+                                       const int32_t* leaf_slot, int x, int y,
+                                       int z) {
+  const int c = cube_index(cube, x, y, z);
+  return c >= 0 ? leaf_slot[c] : -1;
 }
 
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     CastParams p, const int32_t* __restrict__ leaf_slot,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_mask,
-    const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
-    uint32_t* __restrict__ gap, FrameCounters* __restrict__ cnt) {
-  const int i = blockIdx.x * kCastThreads + threadIdx.x;
+    const uint32_t* __restrict__ hit_start,
+    const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
+    FrameCounters* __restrict__ cnt) {
+  __shared__ uint32_t s_key[kLdsSlots];
+  __shared__ uint32_t s_cnt[kLdsSlots];
+  for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
+    s_key[k] = kInvalid;
+    s_cnt[k] = 0;
+  }
+  __syncthreads();
+  const int i = point_of(p, blockIdx.x, threadIdx.x);
   RayDir r;
   int errors = 0;
-  if (i < p.n && ray_of<M>(p, i, r)) {
-    const int nh = cnt->n_hit_rays;
+  if (i >= 0 && ray_of<M>(p, i, r)) {
     Dda s;
     dda_init<M>(p.eye, r, s);
     int px = s.vx, py = s.vy, pz = s.vz;
-    int lkey = 0x7fffffff;  // leaf of the cached slot (packed, any value)
-    int slot = -1;
-    bool first = true;
+    int slot = slot_of(p.cube, leaf_slot, px, py, pz);
     while (dda_step<M>(s, r.L)) {
       // miss on (px, py, pz)
-      const int c = cube_index(p.cube, px, py, pz);
-      if (first || c != lkey) {
-        first = false;
-        lkey = c;
-        slot = c >= 0 ? leaf_slot[c] : -1;
-      }
       if (slot < 0) {
         ++errors;
       } else {
@@ -308,12 +404,27 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
         const uint32_t cell = (uint32_t)slot * 512u + (uint32_t)li;
         const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
         if ((mw >> (li & 31)) & 1u) {
-          const int pos =
-              lower_bound_pair(skeys, svals, 0, nh, cell, (uint32_t)i);
-          if (pos < nh && skeys[pos] == cell) atomicAdd(&gap[pos], 1u);
+          // a voxel holding hits: count the miss directly and mark the
+          // gap before the first later hit of this voxel (its hits are
+          // sorted[s0, s0 + j) in ascending pixel order)
+          const uint32_t s0 = hit_start[cell];
+          const uint32_t j = tiles[2 * (size_t)cell + 1];
+          uint32_t lo = s0, hi = s0 + j;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (svals[mid] < (uint32_t)i)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          if (lo < s0 + j) atomicAdd(&gap[lo], 1u);
+          atomicAdd(&tiles[2 * (size_t)cell], 1u);
+        } else {
+          wave_lds_count(s_key, s_cnt, tiles, cell);
         }
-        atomicAdd(&tiles[2 * (size_t)cell], 1u);
       }
+      if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
+        slot = slot_of(p.cube, leaf_slot, s.vx, s.vy, s.vz);
       px = s.vx;
       py = s.vy;
       pz = s.vz;
@@ -321,6 +432,11 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
   }
   const uint64_t se = wave_sum((uint64_t)errors);
   if ((threadIdx.x & 63) == 0 && se) atomicAdd(&cnt->errors, (int)se);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
+    const uint32_t key = s_key[k];
+    if (key != kInvalid) atomicAdd(&tiles[2 * (size_t)key], s_cnt[k]);
+  }
 }
 
 // ---- map hash ----------------------------------------------------------
@@ -375,32 +491,51 @@ __device__ int map_get_or_insert(const MapView& m, int x, int y, int z,
   return -1;
 }
 
-// First sorted index of `cell` among keys[0, nh).
-__device__ __forceinline__ int lower_bound_key(const uint32_t* keys, int nh,
-                                               uint32_t cell) {
-  int lo = 0, hi = nh;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (keys[mid] < cell)
-      lo = mid + 1;
-    else
-      hi = mid;
+// Leaf index of every discovered tile in the map, or -1 (lookup only, one
+// lane per tile, so the probe latencies overlap instead of serializing
+// inside the update workgroups).
+__global__ __launch_bounds__(256) void k_map_lookup(
+    Cube cube, const int32_t* __restrict__ slot_leaf, MapView m, int tile_cap,
+    const FrameCounters* __restrict__ cnt, int32_t* __restrict__ map_idx) {
+  const int nleaves = min(cnt->n_leaves, tile_cap);
+  const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
+  for (int slot = blockIdx.x * 256 + threadIdx.x; slot < nleaves;
+       slot += gridDim.x * 256) {
+    const int c = slot_leaf[slot];
+    const int lz = c % cube.side;
+    const int ly = (c / cube.side) % cube.side;
+    const int lx = c / (cube.side * cube.side);
+    const unsigned long long key = pack_leaf((lx + cube.cmin[0]) * 8,
+                                             (ly + cube.cmin[1]) * 8,
+                                             (lz + cube.cmin[2]) * 8);
+    unsigned long long h = mix64(key) & mask;
+    int idx = -1;
+    for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
+      const unsigned long long k = m.hash_keys[h];
+      if (k == key) {
+        idx = m.hash_vals[h];
+        break;
+      }
+      if (k == ~0ull) break;
+      h = (h + 1) & mask;
+    }
+    map_idx[slot] = idx;
   }
-  return lo;
 }
 
 template <int M>
 __global__ __launch_bounds__(512) void k_update(
     Cube cube, int32_t* __restrict__ leaf_slot,
     const int32_t* __restrict__ slot_leaf, uint32_t* __restrict__ tiles,
-    uint32_t* __restrict__ hit_mask, const uint32_t* __restrict__ skeys,
-    const uint32_t* __restrict__ gap, MapView m, float hit, float miss,
-    int tile_cap, FrameCounters* __restrict__ cnt) {
+    uint32_t* __restrict__ hit_mask, const uint32_t* __restrict__ hit_start,
+    const uint32_t* __restrict__ gap, const int32_t* __restrict__ map_idx,
+    MapView m, float hit, float miss, int tile_cap,
+    FrameCounters* __restrict__ cnt, uint64_t* __restrict__ part) {
   __shared__ int s_idx;
+  __shared__ uint64_t s_part[8][3];
   const int li = threadIdx.x;
   const int lane = li & 63, wave = li >> 6;
   const int nleaves = min(cnt->n_leaves, tile_cap);
-  const int nh = cnt->n_hit_rays;
   uint64_t n_vox = 0, n_hit = 0, n_mixed = 0;
   for (int slot = blockIdx.x; slot < nleaves; slot += gridDim.x) {
     const int c = slot_leaf[slot];
@@ -414,12 +549,19 @@ __global__ __launch_bounds__(512) void k_update(
     const uint32_t k = tiles[2 * cell];      // misses
     const uint32_t j = tiles[2 * cell + 1];  // hits
     const bool touched = (k | j) != 0;
+    const int known = map_idx[slot];
+    float old_v = 0.0f;
+    uint32_t old_w = 0;
+    if (known >= 0) {  // issue the map loads before the fold
+      old_v = m.values[(size_t)known * 512 + li];
+      old_w = m.masks[(size_t)known * kMaskWords + (li >> 5)];
+    }
     float tv = 0.0f;
     if (touched) {
       if (j == 0) {
         tv = fold<M>(0.0f, miss, k);
       } else {
-        const int s0 = lower_bound_key(skeys, nh, (uint32_t)cell);
+        const uint32_t s0 = hit_start[cell];
         uint32_t used = 0;
         float acc = 0.0f;
         for (uint32_t g = 0; g < j; ++g) {
@@ -432,19 +574,22 @@ __global__ __launch_bounds__(512) void k_update(
       }
     }
     const bool nz = touched && tv != 0.0f;
-    if (li == 0) s_idx = -1;
-    __syncthreads();
     const int any = __syncthreads_or(nz);
     if (any) {
-      if (li == 0) s_idx = map_get_or_insert(m, ox, oy, oz, cnt);
-      __syncthreads();
-      const int idx = s_idx;
+      int idx = known;
+      if (idx < 0) {  // a leaf new to the map
+        if (li == 0) s_idx = map_get_or_insert(m, ox, oy, oz, cnt);
+        __syncthreads();
+        idx = s_idx;
+        old_v = 0.0f;  // fresh pool leaves are zero and inactive
+        old_w = 0;
+      }
       if (idx >= 0) {
         float* mv = m.values + (size_t)idx * 512;
         uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
-        bool act = (mm[li >> 5] >> (li & 31)) & 1u;
+        bool act = (old_w >> (li & 31)) & 1u;
         if (nz) {
-          float v = add<M>(mv[li], tv);
+          float v = add<M>(old_v, tv);
           if (v > m.clamp_max) {
             v = m.clamp_max;
           } else if (v < m.clamp_min) {
@@ -479,9 +624,15 @@ __global__ __launch_bounds__(512) void k_update(
   n_hit = wave_sum(n_hit);
   n_mixed = wave_sum(n_mixed);
   if (lane == 0) {
-    if (n_vox) atomicAdd((unsigned long long*)&cnt->voxels, (unsigned long long)n_vox);
-    if (n_hit) atomicAdd((unsigned long long*)&cnt->hit_voxels, (unsigned long long)n_hit);
-    if (n_mixed) atomicAdd((unsigned long long*)&cnt->mixed_voxels, (unsigned long long)n_mixed);
+    s_part[wave][0] = n_vox;
+    s_part[wave][1] = n_hit;
+    s_part[wave][2] = n_mixed;
+  }
+  __syncthreads();
+  if (li < 3) {
+    uint64_t t = 0;
+    for (int w = 0; w < 8; ++w) t += s_part[w][li];
+    part[(size_t)blockIdx.x * 4 + li] = t;
   }
 }
 
@@ -509,21 +660,24 @@ hipError_t launch_reset_counters(FrameCounters* c, hipStream_t s) {
 hipError_t launch_discover(int rounding, const CastParams& p,
                            const CtxBuffers& b, hipStream_t s) {
   if (p.n <= 0) return hipSuccess;
-  const int grid = (p.n + kCastThreads - 1) / kCastThreads;
+  const int grid = cast_grid(p);
   if (rounding == kDown)
     k_discover<kDown><<<grid, kCastThreads, 0, s>>>(p, b.leaf_slot, b.ray_hit,
-                                                    b.counters);
+                                                    b.part_discover);
   else
     k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(p, b.leaf_slot,
-                                                       b.ray_hit, b.counters);
+                                                       b.ray_hit,
+                                                       b.part_discover);
   return hipGetLastError();
 }
 
-hipError_t launch_compact_leaves(const CtxBuffers& b, int64_t cube_vol,
-                                 int tile_cap, hipStream_t s) {
+hipError_t launch_compact_leaves(const CastParams& p, const CtxBuffers& b,
+                                 int64_t cube_vol, int tile_cap,
+                                 hipStream_t s) {
   const int nblocks = (int)((cube_vol + kScanChunk - 1) / kScanChunk);
   k_scan_count<<<nblocks, 256, 0, s>>>(b.leaf_slot, cube_vol, b.scan_blocks);
-  k_scan_blocks<<<1, 1024, 0, s>>>(b.scan_blocks, nblocks, b.counters);
+  k_scan_blocks<<<1, 1024, 0, s>>>(b.scan_blocks, nblocks, b.part_discover,
+                                   p.n > 0 ? cast_grid(p) : 0, b.counters);
   k_scan_assign<<<nblocks, 256, 0, s>>>(b.leaf_slot, cube_vol, b.scan_blocks,
                                         b.slot_leaf, tile_cap, b.counters);
   return hipGetLastError();
@@ -542,7 +696,7 @@ hipError_t launch_hit_segments(const CtxBuffers& b, int n, int sorted_buf,
   if (n <= 0) return hipSuccess;
   const int grid = min((n + 255) / 256, 2048);
   k_hit_segments<<<grid, 256, 0, s>>>(b.keys[sorted_buf], b.counters, b.tiles,
-                                      b.hit_mask, b.gap);
+                                      b.hit_mask, b.hit_start, b.gap);
   return hipGetLastError();
 }
 
@@ -550,14 +704,14 @@ hipError_t launch_accumulate(int rounding, const CastParams& p,
                              const CtxBuffers& b, int sorted_buf,
                              hipStream_t s) {
   if (p.n <= 0) return hipSuccess;
-  const int grid = (p.n + kCastThreads - 1) / kCastThreads;
+  const int grid = cast_grid(p);
   if (rounding == kDown)
     k_accumulate<kDown><<<grid, kCastThreads, 0, s>>>(
-        p, b.leaf_slot, b.tiles, b.hit_mask, b.keys[sorted_buf],
+        p, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
         b.vals[sorted_buf], b.gap, b.counters);
   else
     k_accumulate<kNearest><<<grid, kCastThreads, 0, s>>>(
-        p, b.leaf_slot, b.tiles, b.hit_mask, b.keys[sorted_buf],
+        p, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
         b.vals[sorted_buf], b.gap, b.counters);
   return hipGetLastError();
 }
@@ -565,17 +719,21 @@ hipError_t launch_accumulate(int rounding, const CastParams& p,
 hipError_t launch_update(int rounding, const Cube& cube, const CtxBuffers& b,
                          const MapView& m, float hit, float miss,
                          int sorted_buf, int tile_cap, hipStream_t s) {
-  const int grid = 2048;
+  (void)sorted_buf;
+  k_map_lookup<<<256, 256, 0, s>>>(cube, b.slot_leaf, m, tile_cap, b.counters,
+                                   b.map_idx);
+  const int grid = kUpdateGrid;
   if (rounding == kDown)
     k_update<kDown><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
-                                         b.tiles, b.hit_mask,
-                                         b.keys[sorted_buf], b.gap, m, hit,
-                                         miss, tile_cap, b.counters);
+                                         b.tiles, b.hit_mask, b.hit_start,
+                                         b.gap, b.map_idx, m, hit, miss,
+                                         tile_cap, b.counters, b.part_update);
   else
     k_update<kNearest><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
-                                            b.tiles, b.hit_mask,
-                                            b.keys[sorted_buf], b.gap, m, hit,
-                                            miss, tile_cap, b.counters);
+                                            b.tiles, b.hit_mask, b.hit_start,
+                                            b.gap, b.map_idx, m, hit, miss,
+                                            tile_cap, b.counters,
+                                            b.part_update);
   return hipGetLastError();
 }
 

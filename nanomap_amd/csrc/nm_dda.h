@@ -55,6 +55,14 @@ NM_HD bool ray_from_point(const float* R, float px, float py, float pz,
   return true;
 }
 
+// bx ? a : (by ? b : c) on values, written with bit masks so the compiler
+// keeps it a register select.
+NM_HD double fmin_sel(bool bx, double a, bool by, double b, double c) {
+  const uint64_t mx = 0ull - (uint64_t)bx, my = 0ull - (uint64_t)(by && !bx);
+  const uint64_t mz = ~(mx | my);
+  return double_of((bits_of(a) & mx) | (bits_of(b) & my) | (bits_of(c) & mz));
+}
+
 // Voxel DDA state.  Axes are kept in named scalars so the per-step
 // argmin compiles to selects rather than indexed register access.
 struct Dda {
@@ -94,18 +102,24 @@ NM_HD void dda_init(const double* eye, const RayDir& r, Dda& s) {
   s.t = 0.0;
 }
 
-// One DDA::step(); returns t <= L.  Branch-free: one rounded add.
+// One DDA::step(); returns t <= L.  Branch-free: one rounded add.  The
+// state is read into scalars and written back unconditionally, so the
+// axis choice is a select between register values (a select between two
+// struct fields can otherwise become a load from a selected address, which
+// pins the state to scratch memory).
 template <int M>
 NM_HD bool dda_step(Dda& s, double L) {
-  const bool bx = (s.nx < s.ny) && (s.nx < s.nz);
-  const bool by = !bx && (s.ny < s.nz);
+  const double nx = s.nx, ny = s.ny, nz = s.nz;
+  const double ddx = s.dx, ddy = s.dy, ddz = s.dz;
+  const bool bx = (nx < ny) && (nx < nz);
+  const bool by = !bx && (ny < nz);
   const bool bz = !bx && !by;
-  const double t = bx ? s.nx : (by ? s.ny : s.nz);
-  const double dt = bx ? s.dx : (by ? s.dy : s.dz);
+  const double t = fmin_sel(bx, nx, by, ny, nz);
+  const double dt = fmin_sel(bx, ddx, by, ddy, ddz);
   const double nt = add<M>(t, dt);
-  s.nx = bx ? nt : s.nx;
-  s.ny = by ? nt : s.ny;
-  s.nz = bz ? nt : s.nz;
+  s.nx = bx ? nt : nx;
+  s.ny = by ? nt : ny;
+  s.nz = bz ? nt : nz;
   s.vx += bx ? s.sx : 0;
   s.vy += by ? s.sy : 0;
   s.vz += bz ? s.sz : 0;

@@ -185,6 +185,11 @@ NM_HD double abs_(double x) { return x < 0 ? -x : x; }
 template <int M>
 NM_HD float fold(float v, float a, uint32_t k) {
   if (a == 0.0f) return k ? add<M>(v, a) : v;
+  // short runs: plain rounded steps are cheaper than the run analysis
+  while (k > 0 && k <= 24) {
+    v = add<M>(v, a);
+    --k;
+  }
   while (k > 0) {
     const double av = abs_((double)v), aa = abs_((double)a);
     // Single steps while |v| is small relative to the increment, so every

@@ -71,6 +71,8 @@ struct CastParams {
   int n;                       // points (upper bound on rays)
   int step;
   int width;
+  int height;
+  int tiled;                   // 1: points form a width x height image
   int use_rays;                // 1: cast rays4 instead of points
 };
 
@@ -78,8 +80,10 @@ struct CastParams {
 struct CtxBuffers {
   int32_t* leaf_slot;   // [cube_cap]  -1 free, -2 discovered, >= 0 slot
   int32_t* slot_leaf;   // [tile_cap]  cube index of each slot
+  int32_t* map_idx;     // [tile_cap]  map leaf of each slot, -1 if new
   uint32_t* tiles;      // [tile_cap * 512 * 2]  (miss, hit) per voxel
   uint32_t* hit_mask;   // [tile_cap * 16]  voxels holding a hit
+  uint32_t* hit_start;  // [tile_cap * 512] first sorted hit of a hit voxel
   uint32_t* ray_hit;    // [max_points]  cube cell of each ray's hit
   uint32_t* keys[2];    // [max_points]  radix sort buffers
   uint32_t* vals[2];
@@ -87,6 +91,8 @@ struct CtxBuffers {
   uint32_t* radix_hist; // [256 * max_radix_blocks]
   int32_t* scan_blocks; // [cube_cap / kScanChunk + 1]
   FrameCounters* counters;
+  uint64_t* part_discover;  // [discover blocks * 4] per-block counter sums
+  uint64_t* part_update;    // [kUpdateGrid * 4] voxels, hit, mixed
   unsigned char* cloud; // [max_points * 16] staging for host clouds
   float* filter_rays;   // [max_points * 4]
   float* filter_tmp;    // [max_points * 4]
@@ -94,14 +100,16 @@ struct CtxBuffers {
 };
 
 constexpr int kScanChunk = 4096;
+constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
 
 // ---- launchers (nm_cast.hip, nm_radix.hip, nm_filter.hip) -------------
 hipError_t launch_reset_counters(FrameCounters* c, hipStream_t s);
 hipError_t launch_discover(int rounding, const CastParams& p,
                            const CtxBuffers& b, hipStream_t s);
-hipError_t launch_compact_leaves(const CtxBuffers& b, int64_t cube_vol,
-                                 int tile_cap, hipStream_t s);
+hipError_t launch_compact_leaves(const CastParams& p, const CtxBuffers& b,
+                                 int64_t cube_vol, int tile_cap,
+                                 hipStream_t s);
 hipError_t launch_hit_keys(const CastParams& p, const CtxBuffers& b,
                            hipStream_t s);
 hipError_t launch_hit_segments(const CtxBuffers& b, int n, int sorted_buf,
@@ -120,6 +128,7 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
                             int bits, uint32_t* hist, int* out_buf,
                             hipStream_t s);
 int radix_max_blocks(int n);
+int radix_hist_words(int n);  // histogram buffer size for n keys
 
 // Voxel filter: bins points by voxel, writes one ray per bin into
 // b.filter_rays (count -> counters->n_filter_rays).  origin: fractional

@@ -1,9 +1,11 @@
 // nm_radix.hip -- stable LSD radix sort of (u32 key, u32 value) pairs.
 //
 // Used to group hit events by voxel (in ascending pixel order) and to bin
-// points by voxel in the voxel filter.  8-bit digits; each pass is
+// points by voxel in the voxel filter.  Digits of D = 8..11 bits, chosen so
+// the pass count is minimal (27 key bits: 3 passes of 9); each pass is
 //   hist:    per-tile digit histogram (LDS atomics), digit-major layout;
-//   scan:    one workgroup turns the histogram into exclusive offsets;
+//   rowscan: one workgroup per digit: exclusive prefix over the tiles and
+//            the digit total (the scatter scans the 2^D totals in LDS);
 //   scatter: stable rank of every key inside its tile, computed with
 //            wavefront ballots on the 8 digit bits (a 64-lane match), a
 //            per-wave digit count in LDS and a running per-digit count
@@ -18,78 +20,106 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kItems = kRadixTile / kThreads;  // 8
-constexpr int kBins = 256;
 constexpr int kWaves = kThreads / 64;
 
+template <int D>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t* __restrict__ keys, int n, int shift,
     uint32_t* __restrict__ hist, int nblocks) {
+  constexpr int kBins = 1 << D;
   __shared__ uint32_t h[kBins];
   const int tid = threadIdx.x;
-  h[tid] = 0;
+  for (int b = tid; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
   const int base = blockIdx.x * kRadixTile;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int i = base + j * kThreads + tid;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xff], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1)], 1u);
   }
   __syncthreads();
-  hist[tid * nblocks + blockIdx.x] = h[tid];
+  for (int b = tid; b < kBins; b += kThreads) hist[b * nblocks + blockIdx.x] = h[b];
 }
 
-// Exclusive scan of hist[0, total) by one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist,
-                                                     int total) {
-  __shared__ uint32_t wsum[16];
-  const int tid = threadIdx.x;
-  const int per = (total + 1023) / 1024;
-  const int b0 = tid * per;
-  const int b1 = min(total, b0 + per);
-  uint32_t local = 0;
-  for (int i = b0; i < b1; ++i) local += hist[i];
-  // inclusive scan of `local` across the block
-  const int lane = tid & 63, wave = tid >> 6;
-  uint32_t x = local;
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) wsum[wave] = x;
-  __syncthreads();
-  if (tid < 16) {
-    uint32_t w = wsum[tid];
-    for (int o = 1; o < 16; o <<= 1) {
-      const uint32_t y = __shfl_up(w, o, 16);
-      if (tid >= o) w += y;
+// Row scan: workgroup d turns row d of the digit-major histogram (the
+// counts of digit d in every tile) into exclusive prefixes and writes the
+// row total to totals[d].  Rows are short (one entry per 2048 keys), so
+// each workgroup loops over its row in coalesced 256-wide chunks.
+__global__ __launch_bounds__(256) void k_radix_rowscan(
+    uint32_t* __restrict__ hist, int nblocks, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t ws[4];
+  uint32_t* row = hist + (size_t)blockIdx.x * nblocks;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t run = 0;
+  for (int c0 = 0; c0 < nblocks; c0 += 256) {
+    const int c = c0 + tid;
+    const uint32_t v = c < nblocks ? row[c] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    wsum[tid] = w;
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; ++w) pre += ws[w];
+    if (c < nblocks) row[c] = run + pre + x - v;
+    run += ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
   }
-  __syncthreads();
-  uint32_t run = x - local + (wave ? wsum[wave - 1] : 0u);
-  for (int i = b0; i < b1; ++i) {
-    const uint32_t v = hist[i];
-    hist[i] = run;
-    run += v;
-  }
+  if (tid == 0) totals[blockIdx.x] = run;
 }
 
+template <int D>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
-    int shift, const uint32_t* __restrict__ hist, int nblocks) {
+    int shift, const uint32_t* __restrict__ hist, int nblocks,
+    const uint32_t* __restrict__ totals) {
+  constexpr int kBins = 1 << D;
   __shared__ uint32_t running[kBins];
   __shared__ uint32_t offset[kBins];
   __shared__ uint32_t wcnt[kWaves][kBins];
+  __shared__ uint32_t ws[kWaves];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  running[tid] = 0;
-  offset[tid] = hist[tid * nblocks + blockIdx.x];
+  // digit base = exclusive scan of the digit totals, in LDS
+  {
+    constexpr int kPer = kBins / kThreads > 0 ? kBins / kThreads : 1;
+    uint32_t loc[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int b = tid * kPer + q;
+      loc[q] = b < kBins ? totals[b] : 0u;
+      sum += loc[q];
+    }
+    uint32_t x = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    uint32_t pre = x - sum;
+    for (int w = 0; w < wave; ++w) pre += ws[w];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int b = tid * kPer + q;
+      if (b < kBins) {
+        running[b] = 0;
+        offset[b] = pre + hist[b * nblocks + blockIdx.x];
+      }
+      pre += loc[q];
+    }
+  }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int base = blockIdx.x * kRadixTile;
   for (int j = 0; j < kItems; ++j) {
+    for (int b = tid; b < kBins; b += kThreads) {
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+      for (int w = 0; w < kWaves; ++w) wcnt[w][b] = 0;
+    }
     __syncthreads();
     const int i = base + j * kThreads + tid;
     const bool valid = i < n;
@@ -97,11 +127,11 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     if (valid) {
       key = kin[i];
       val = vin[i];
-      d = (key >> shift) & 0xff;
+      d = (key >> shift) & (kBins - 1);
     }
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < D; ++b) {
       const bool bit = (d >> b) & 1;
       const uint64_t bb = __ballot(valid && bit);
       peers &= bit ? bb : ~bb;
@@ -116,16 +146,30 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       vout[pos] = val;
     }
     __syncthreads();
-    uint32_t tot = 0;
+    for (int b = tid; b < kBins; b += kThreads) {
+      uint32_t tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) tot += wcnt[w][tid];
-    running[tid] += tot;
+      for (int w = 0; w < kWaves; ++w) tot += wcnt[w][b];
+      running[b] += tot;
+    }
   }
+}
+
+template <int D>
+void radix_pass(uint32_t* kin, uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                int n, int shift, uint32_t* hist, int nblocks,
+                hipStream_t s) {
+  uint32_t* totals = hist + (size_t)(1 << D) * nblocks;
+  k_radix_hist<D><<<nblocks, kThreads, 0, s>>>(kin, n, shift, hist, nblocks);
+  k_radix_rowscan<<<1 << D, 256, 0, s>>>(hist, nblocks, totals);
+  k_radix_scatter<D><<<nblocks, kThreads, 0, s>>>(kin, vin, kout, vout, n,
+                                                  shift, hist, nblocks, totals);
 }
 
 }  // namespace
 
 int radix_max_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
+int radix_hist_words(int n) { return 2048 * (radix_max_blocks(n) + 2); }
 
 hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
                             int bits, uint32_t* hist, int* out_buf,
@@ -134,13 +178,17 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
   *out_buf = 0;
   if (n <= 0) return hipSuccess;
   const int nblocks = radix_max_blocks(n);
-  for (int shift = 0; shift < bits; shift += 8) {
-    k_radix_hist<<<nblocks, kThreads, 0, s>>>(keys[cur], n, shift, hist,
-                                              nblocks);
-    k_radix_scan<<<1, 1024, 0, s>>>(hist, kBins * nblocks);
-    k_radix_scatter<<<nblocks, kThreads, 0, s>>>(
-        keys[cur], vals[cur], keys[cur ^ 1], vals[cur ^ 1], n, shift, hist,
-        nblocks);
+  const int passes = (bits + 10) / 11;
+  const int D = (bits + passes - 1) / passes < 8 ? 8 : (bits + passes - 1) / passes;
+  for (int shift = 0; shift < bits; shift += D) {
+    uint32_t *ki = keys[cur], *vi = vals[cur], *ko = keys[cur ^ 1],
+             *vo = vals[cur ^ 1];
+    switch (D) {
+      case 8: radix_pass<8>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
+      case 9: radix_pass<9>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
+      case 10: radix_pass<10>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
+      default: radix_pass<11>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
+    }
     cur ^= 1;
   }
   *out_buf = cur;
