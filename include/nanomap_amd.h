@@ -118,7 +118,11 @@ int nm_map_leaf_count(nm_map* map, int64_t* count);
  * Leaves are in allocation order.  Synchronous. */
 int nm_map_download(nm_map* map, int32_t* origins, float* values,
                     uint32_t* masks);
-/* Device pointers of the map arrays (for kernels and zero-copy readers). */
+/* Device pointers of the map arrays (for kernels and zero-copy readers).
+ * The leaf pool may hold holes: slots reserved for a leaf that was never
+ * inserted (no voxel of it changed); a hole's origin x is INT32_MIN and its
+ * values and mask are zero.  Holes are skipped by nm_map_leaf_count and
+ * nm_map_download. */
 int nm_map_device_arrays(nm_map* map, void** values, void** masks,
                          void** origins, void** hash_keys, void** hash_vals,
                          int64_t* hash_capacity);

@@ -8,16 +8,22 @@
 // every count stays on the device, kernels are sized by worst-case
 // capacities allocated once per context, and a frame is a fixed sequence
 // of launches on one stream with no host synchronization.
+#include <limits.h>
 #include <math.h>
+
 #include <algorithm>
 #include <cmath>
 #include <stdio.h>
 #include <string.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "nm_fp.h"
+#include <climits>
+#include <cstring>
+#include <utility>
 #include "nm_internal.h"
 
 struct nm_ctx {
@@ -158,7 +164,7 @@ void free_ctx(nm_ctx* c) {
   }
   for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
   nm::CtxBuffers& b = c->b;
-  void* ptrs[] = {b.leaf_slot,  b.slot_leaf, b.map_idx,  b.tiles,      b.hit_mask, b.hit_start,
+  void* ptrs[] = {b.leaf_slot,  b.slot_leaf, b.map_idx, b.map_new,  b.tiles,      b.hit_mask, b.hit_start,
                   b.ray_hit,    b.keys[0],    b.keys[1],    b.vals[0],
                   b.vals[1],    b.gap,        b.radix_hist, b.scan_blocks,
                   b.counters,   b.part_discover, b.part_update,
@@ -219,6 +225,7 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   A(dalloc(&b.leaf_slot, (size_t)c->cube_cap));
   A(dalloc(&b.slot_leaf, (size_t)c->tile_cap));
   A(dalloc(&b.map_idx, (size_t)c->tile_cap));
+  A(dalloc(&b.map_new, (size_t)c->tile_cap));
   A(dalloc(&b.tiles, (size_t)c->tile_cap * 1024));
   A(dalloc(&b.hit_mask, (size_t)c->tile_cap * nm::kMaskWords));
   A(dalloc(&b.hit_start, (size_t)c->tile_cap * 512));
@@ -296,7 +303,7 @@ int nm_map_create(int device, float grid_res, float clamp_min,
   A(dalloc(&v.values, (size_t)leaf_capacity * 512));
   A(dalloc(&v.masks, (size_t)leaf_capacity * nm::kMaskWords));
   A(dalloc(&v.origins, (size_t)leaf_capacity * 3));
-  A(dalloc(&v.leaf_count, 1));
+  A(dalloc(&v.leaf_count, 2));
   if (e == hipSuccess) e = nm::launch_map_clear(v, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -328,32 +335,65 @@ int nm_map_clear(nm_map* m, void* stream) {
   return NM_OK;
 }
 
-int nm_map_leaf_count(nm_map* m, int64_t* count) {
-  if (!m || !count) return fail(NM_EINVAL, "nm_map_leaf_count: null argument");
+// Pool slots in use and holes among them (slots reserved for a leaf that
+// was then not inserted because none of its voxels changed).
+static int map_slots(nm_map* m, int64_t* slots, int64_t* holes) {
   NM_HIP(hipSetDevice(m->device));
   NM_HIP(hipStreamSynchronize(m->stream));
-  int32_t n = 0;
-  NM_HIP(hipMemcpy(&n, m->v.leaf_count, sizeof(n), hipMemcpyDeviceToHost));
-  *count = std::min<int64_t>(n, m->v.leaf_cap);
+  int32_t n[2] = {0, 0};
+  NM_HIP(hipMemcpy(n, m->v.leaf_count, sizeof(n), hipMemcpyDeviceToHost));
+  *slots = std::min<int64_t>(n[0], m->v.leaf_cap);
+  *holes = n[1];
+  return NM_OK;
+}
+
+int nm_map_leaf_count(nm_map* m, int64_t* count) {
+  if (!m || !count) return fail(NM_EINVAL, "nm_map_leaf_count: null argument");
+  int64_t slots = 0, holes = 0;
+  int st = map_slots(m, &slots, &holes);
+  if (st != NM_OK) return st;
+  *count = slots - holes;
   return NM_OK;
 }
 
 int nm_map_download(nm_map* m, int32_t* origins, float* values,
                     uint32_t* masks) {
-  int64_t n = 0;
-  int st = nm_map_leaf_count(m, &n);
+  if (!m) return fail(NM_EINVAL, "nm_map_download: null map");
+  int64_t slots = 0, holes = 0;
+  int st = map_slots(m, &slots, &holes);
   if (st != NM_OK) return st;
-  if (n == 0) return NM_OK;
-  if (origins)
-    NM_HIP(hipMemcpy(origins, m->v.origins, (size_t)n * 3 * sizeof(int32_t),
-                     hipMemcpyDeviceToHost));
-  if (values)
-    NM_HIP(hipMemcpy(values, m->v.values, (size_t)n * 512 * sizeof(float),
-                     hipMemcpyDeviceToHost));
-  if (masks)
-    NM_HIP(hipMemcpy(masks, m->v.masks,
-                     (size_t)n * nm::kMaskWords * sizeof(uint32_t),
-                     hipMemcpyDeviceToHost));
+  if (slots == 0) return NM_OK;
+  // runs [begin, end) of consecutive non-hole slots
+  std::vector<std::pair<int64_t, int64_t>> runs;
+  std::vector<int32_t> all((size_t)slots * 3);
+  NM_HIP(hipMemcpy(all.data(), m->v.origins, all.size() * sizeof(int32_t),
+                   hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < slots;) {
+    if (all[3 * i] == INT_MIN) {
+      ++i;
+      continue;
+    }
+    int64_t j = i;
+    while (j < slots && all[3 * j] != INT_MIN) ++j;
+    runs.emplace_back(i, j);
+    i = j;
+  }
+  int64_t out = 0;
+  for (const auto& r : runs) {
+    const int64_t n = r.second - r.first;
+    if (origins)
+      memcpy(origins + 3 * out, all.data() + 3 * r.first,
+             (size_t)n * 3 * sizeof(int32_t));
+    if (values)
+      NM_HIP(hipMemcpy(values + out * 512, m->v.values + r.first * 512,
+                       (size_t)n * 512 * sizeof(float), hipMemcpyDeviceToHost));
+    if (masks)
+      NM_HIP(hipMemcpy(masks + out * nm::kMaskWords,
+                       m->v.masks + r.first * nm::kMaskWords,
+                       (size_t)n * nm::kMaskWords * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost));
+    out += n;
+  }
   return NM_OK;
 }
 
@@ -530,10 +570,10 @@ int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out) {
   out->errors = c.errors;
   out->overflow = c.overflow;
   if (ctx->last_map) {
-    int32_t n = 0;
-    NM_HIP(hipMemcpy(&n, ctx->last_map->v.leaf_count, sizeof(n),
+    int32_t n[2] = {0, 0};
+    NM_HIP(hipMemcpy(n, ctx->last_map->v.leaf_count, sizeof(n),
                      hipMemcpyDeviceToHost));
-    out->map_leaves = n;
+    out->map_leaves = std::min<int64_t>(n[0], ctx->last_map->v.leaf_cap) - n[1];
   }
   return NM_OK;
 }

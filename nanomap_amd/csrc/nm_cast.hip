@@ -28,6 +28,8 @@
 //
 // Counts are integers, so the result does not depend on the order in
 // which lanes run: the map is bit-identical to the sequential CPU fold.
+#include <climits>
+
 #include "nm_dda.h"
 #include "nm_internal.h"
 
@@ -292,7 +294,8 @@ __global__ __launch_bounds__(256) void k_hit_keys(
 __global__ __launch_bounds__(256) void k_hit_segments(
     const uint32_t* __restrict__ keys, const FrameCounters* __restrict__ cnt,
     uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
-    uint32_t* __restrict__ hit_start, uint32_t* __restrict__ gap) {
+    uint32_t* __restrict__ hit_start, uint32_t* __restrict__ gap,
+    const int32_t* __restrict__ slot_leaf, int32_t* __restrict__ leaf_slot) {
   const int nh = cnt->n_hit_rays;
   for (int e = blockIdx.x * 256 + threadIdx.x; e < nh;
        e += gridDim.x * 256) {
@@ -313,6 +316,9 @@ __global__ __launch_bounds__(256) void k_hit_segments(
     hit_start[key] = (uint32_t)e;
     const uint32_t slot = key >> 9, li = key & 511;
     atomicOr(&hit_mask[slot * kMaskWords + (li >> 5)], 1u << (li & 31));
+    // flag the leaf in the cube table so the accumulate pass reads its hit
+    // mask only inside leaves that hold hits
+    atomicOr(&leaf_slot[slot_leaf[slot]], kSlotHasHits);
   }
 }
 
@@ -321,15 +327,18 @@ __global__ __launch_bounds__(256) void k_hit_segments(
 // voxels for tens of thousands of visits, so the table turns almost every
 // visit into an LDS atomic and leaves one global atomic per distinct voxel
 // per workgroup, at flush.  A visit that finds no free slot within kProbe
-// probes falls back to a global atomic.
+// probes falls back to a global atomic.  Two unrolled probes measured
+// fastest: longer probe loops diverge across the wavefront and cost more
+// than the rare global fallback; 2048 or 8192 slots lose to 4096 (more
+// collisions, or less occupancy).
 constexpr int kLdsSlots = 4096;
-constexpr int kProbe = 16;
+constexpr int kProbe = 2;
 
 __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
                                           uint32_t* tiles, uint32_t cell,
                                           uint32_t inc) {
   uint32_t h = (cell * 2654435761u) >> 20;  // 12-bit multiplicative hash
-#pragma unroll 1
+#pragma unroll
   for (int q = 0; q < kProbe; ++q) {
     const uint32_t k = s_key[h];
     if (k == cell) {
@@ -366,6 +375,7 @@ __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
   }
 }
 
+// Cube-table entry of the leaf holding (x,y,z): slot | kSlotHasHits, or -1.
 __device__ __forceinline__ int slot_of(const Cube& cube,
                                        const int32_t* leaf_slot, int x, int y,
                                        int z) {
@@ -394,16 +404,21 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     Dda s;
     dda_init<M>(p.eye, r, s);
     int px = s.vx, py = s.vy, pz = s.vz;
-    int slot = slot_of(p.cube, leaf_slot, px, py, pz);
+    int entry = slot_of(p.cube, leaf_slot, px, py, pz);
     while (dda_step<M>(s, r.L)) {
       // miss on (px, py, pz)
-      if (slot < 0) {
+      if (entry < 0) {
         ++errors;
       } else {
+        const int slot = entry & ~kSlotHasHits;
         const int li = local_of(px, py, pz);
         const uint32_t cell = (uint32_t)slot * 512u + (uint32_t)li;
-        const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
-        if ((mw >> (li & 31)) & 1u) {
+        bool hit_voxel = false;
+        if (entry & kSlotHasHits) {
+          const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
+          hit_voxel = (mw >> (li & 31)) & 1u;
+        }
+        if (hit_voxel) {
           // a voxel holding hits: count the miss directly and mark the
           // gap before the first later hit of this voxel (its hits are
           // sorted[s0, s0 + j) in ascending pixel order)
@@ -424,7 +439,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
         }
       }
       if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
-        slot = slot_of(p.cube, leaf_slot, s.vx, s.vy, s.vz);
+        entry = slot_of(p.cube, leaf_slot, s.vx, s.vy, s.vz);
       px = s.vx;
       py = s.vy;
       pz = s.vz;
@@ -456,71 +471,89 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
   return k;
 }
 
-// Leaf index of the leaf with origin (x,y,z), inserting it if absent;
-// -1 if the pool or the table is full.  Within one update kernel every
-// leaf is handled by exactly one workgroup, so an inserted key's value is
-// written before any later kernel reads it.
-__device__ int map_get_or_insert(const MapView& m, int x, int y, int z,
-                                 FrameCounters* cnt) {
+// Map leaf of every discovered tile, one lane per tile so the probe
+// latencies overlap: map_idx = leaf index, or -1 if the leaf is new to the
+// map.  A new leaf gets a pool index reserved here, in tile order (one
+// atomic per wavefront), and its hash insertion point; the update inserts it
+// only if one of its voxels changes (Handler.cpp:116-118 touches the map
+// only for nonzero temp values), otherwise the reserved index is a hole.
+__global__ __launch_bounds__(256) void k_map_lookup(
+    Cube cube, const int32_t* __restrict__ slot_leaf, MapView m, int tile_cap,
+    FrameCounters* __restrict__ cnt, int32_t* __restrict__ map_idx,
+    int32_t* __restrict__ map_new) {
+  const int nleaves = min(cnt->n_leaves, tile_cap);
+  const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
+  const int lane = threadIdx.x & 63;
+  for (int base = blockIdx.x * 256; base < nleaves; base += gridDim.x * 256) {
+    const int slot = base + threadIdx.x;
+    int idx = -1;
+    bool unknown = false;
+    if (slot < nleaves) {
+      const int c = slot_leaf[slot];
+      const int lz = c % cube.side;
+      const int ly = (c / cube.side) % cube.side;
+      const int lx = c / (cube.side * cube.side);
+      const unsigned long long key = pack_leaf((lx + cube.cmin[0]) * 8,
+                                               (ly + cube.cmin[1]) * 8,
+                                               (lz + cube.cmin[2]) * 8);
+      unsigned long long h = mix64(key) & mask;
+      unknown = true;
+      for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
+        const unsigned long long k = m.hash_keys[h];
+        if (k == key) {
+          idx = m.hash_vals[h];
+          unknown = false;
+          break;
+        }
+        if (k == ~0ull) break;
+        h = (h + 1) & mask;
+      }
+    }
+    // reserve pool indices for the unknown leaves of this wavefront
+    const uint64_t want = __ballot(unknown);
+    int first = 0;
+    if (want) {
+      if (lane == (int)__builtin_ctzll(want))
+        first = atomicAdd(m.leaf_count, (int)__popcll(want));
+      first = __shfl(first, (int)__builtin_ctzll(want), 64);
+    }
+    if (slot < nleaves) {
+      map_idx[slot] = idx;
+      int reserved = -1;
+      if (unknown) {
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        reserved = first + (int)__popcll(want & lt);
+        if (reserved >= m.leaf_cap) {
+          atomicAdd(&cnt->overflow, 1);
+          reserved = -1;
+        }
+      }
+      map_new[slot] = reserved;
+    }
+  }
+}
+
+// Insert leaf `idx` (reserved by k_map_lookup) under the key of (x,y,z);
+// false if the table is full.
+__device__ bool map_insert(const MapView& m, int x, int y, int z, int idx) {
   const unsigned long long key = pack_leaf(x, y, z);
   const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
   unsigned long long h = mix64(key) & mask;
   for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
     const unsigned long long k = m.hash_keys[h];
-    if (k == key) return m.hash_vals[h];
     if (k == ~0ull) {
       const unsigned long long old = atomicCAS(&m.hash_keys[h], ~0ull, key);
       if (old == ~0ull) {
-        const int idx = atomicAdd(m.leaf_count, 1);
-        if (idx >= m.leaf_cap) {
-          m.hash_vals[h] = -1;
-          atomicAdd(&cnt->overflow, 1);
-          return -1;
-        }
+        m.hash_vals[h] = idx;
         m.origins[3 * idx + 0] = x;
         m.origins[3 * idx + 1] = y;
         m.origins[3 * idx + 2] = z;
-        m.hash_vals[h] = idx;
-        return idx;
+        return true;
       }
-      if (old == key) return m.hash_vals[h];
     }
     h = (h + 1) & mask;
   }
-  atomicAdd(&cnt->overflow, 1);
-  return -1;
-}
-
-// Leaf index of every discovered tile in the map, or -1 (lookup only, one
-// lane per tile, so the probe latencies overlap instead of serializing
-// inside the update workgroups).
-__global__ __launch_bounds__(256) void k_map_lookup(
-    Cube cube, const int32_t* __restrict__ slot_leaf, MapView m, int tile_cap,
-    const FrameCounters* __restrict__ cnt, int32_t* __restrict__ map_idx) {
-  const int nleaves = min(cnt->n_leaves, tile_cap);
-  const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
-  for (int slot = blockIdx.x * 256 + threadIdx.x; slot < nleaves;
-       slot += gridDim.x * 256) {
-    const int c = slot_leaf[slot];
-    const int lz = c % cube.side;
-    const int ly = (c / cube.side) % cube.side;
-    const int lx = c / (cube.side * cube.side);
-    const unsigned long long key = pack_leaf((lx + cube.cmin[0]) * 8,
-                                             (ly + cube.cmin[1]) * 8,
-                                             (lz + cube.cmin[2]) * 8);
-    unsigned long long h = mix64(key) & mask;
-    int idx = -1;
-    for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
-      const unsigned long long k = m.hash_keys[h];
-      if (k == key) {
-        idx = m.hash_vals[h];
-        break;
-      }
-      if (k == ~0ull) break;
-      h = (h + 1) & mask;
-    }
-    map_idx[slot] = idx;
-  }
+  return false;
 }
 
 template <int M>
@@ -529,10 +562,11 @@ __global__ __launch_bounds__(512) void k_update(
     const int32_t* __restrict__ slot_leaf, uint32_t* __restrict__ tiles,
     uint32_t* __restrict__ hit_mask, const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ gap, const int32_t* __restrict__ map_idx,
-    MapView m, float hit, float miss, int tile_cap,
-    FrameCounters* __restrict__ cnt, uint64_t* __restrict__ part) {
-  __shared__ int s_idx;
+    const int32_t* __restrict__ map_new, MapView m, float hit, float miss,
+    int tile_cap, FrameCounters* __restrict__ cnt,
+    uint64_t* __restrict__ part) {
   __shared__ uint64_t s_part[8][3];
+  __shared__ int s_new;
   const int li = threadIdx.x;
   const int lane = li & 63, wave = li >> 6;
   const int nleaves = min(cnt->n_leaves, tile_cap);
@@ -575,15 +609,25 @@ __global__ __launch_bounds__(512) void k_update(
     }
     const bool nz = touched && tv != 0.0f;
     const int any = __syncthreads_or(nz);
-    if (any) {
-      int idx = known;
-      if (idx < 0) {  // a leaf new to the map
-        if (li == 0) s_idx = map_get_or_insert(m, ox, oy, oz, cnt);
-        __syncthreads();
-        idx = s_idx;
-        old_v = 0.0f;  // fresh pool leaves are zero and inactive
-        old_w = 0;
+    int idx = known;
+    if (known < 0) {  // a leaf new to the map: its pool leaf is fresh (zero)
+      if (li == 0) {
+        const int r = map_new[slot];
+        bool ok = false;
+        if (r >= 0) {
+          ok = any && map_insert(m, ox, oy, oz, r);
+          if (any && !ok) atomicAdd(&cnt->overflow, 1);
+          if (!ok) {  // the reserved pool leaf stays a (zero) hole
+            m.origins[3 * r] = INT_MIN;
+            atomicAdd(m.leaf_count + 1, 1);
+          }
+        }
+        s_new = ok ? r : -1;
       }
+      __syncthreads();
+      idx = s_new;
+    }
+    if (any) {
       if (idx >= 0) {
         float* mv = m.values + (size_t)idx * 512;
         uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
@@ -647,7 +691,10 @@ __global__ void k_map_clear(MapView m) {
     m.values[v] = 0.0f;
   for (int64_t w = t; w < (int64_t)m.leaf_cap * kMaskWords; w += stride)
     m.masks[w] = 0;
-  if (t == 0) *m.leaf_count = 0;
+  if (t == 0) {
+    m.leaf_count[0] = 0;
+    m.leaf_count[1] = 0;
+  }
 }
 
 }  // namespace
@@ -696,7 +743,8 @@ hipError_t launch_hit_segments(const CtxBuffers& b, int n, int sorted_buf,
   if (n <= 0) return hipSuccess;
   const int grid = min((n + 255) / 256, 2048);
   k_hit_segments<<<grid, 256, 0, s>>>(b.keys[sorted_buf], b.counters, b.tiles,
-                                      b.hit_mask, b.hit_start, b.gap);
+                                      b.hit_mask, b.hit_start, b.gap,
+                                      b.slot_leaf, b.leaf_slot);
   return hipGetLastError();
 }
 
@@ -721,18 +769,19 @@ hipError_t launch_update(int rounding, const Cube& cube, const CtxBuffers& b,
                          int sorted_buf, int tile_cap, hipStream_t s) {
   (void)sorted_buf;
   k_map_lookup<<<256, 256, 0, s>>>(cube, b.slot_leaf, m, tile_cap, b.counters,
-                                   b.map_idx);
+                                   b.map_idx, b.map_new);
   const int grid = kUpdateGrid;
   if (rounding == kDown)
     k_update<kDown><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
                                          b.tiles, b.hit_mask, b.hit_start,
-                                         b.gap, b.map_idx, m, hit, miss,
-                                         tile_cap, b.counters, b.part_update);
+                                         b.gap, b.map_idx, b.map_new, m, hit,
+                                         miss, tile_cap, b.counters,
+                                         b.part_update);
   else
     k_update<kNearest><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
                                             b.tiles, b.hit_mask, b.hit_start,
-                                            b.gap, b.map_idx, m, hit, miss,
-                                            tile_cap, b.counters,
+                                            b.gap, b.map_idx, b.map_new, m,
+                                            hit, miss, tile_cap, b.counters,
                                             b.part_update);
   return hipGetLastError();
 }

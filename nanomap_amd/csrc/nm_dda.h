@@ -116,7 +116,9 @@ NM_HD bool dda_step(Dda& s, double L) {
   const bool bz = !bx && !by;
   const double t = fmin_sel(bx, nx, by, ny, nz);
   const double dt = fmin_sel(bx, ddx, by, ddy, ddz);
-  const double nt = add<M>(t, dt);
+  // t >= 0 and dt > 0 (next times start at 0 + a non-negative product and
+  // grow by |1/d|), so the round-down sum needs no sign or zero cases
+  const double nt = M == kDown ? add_down_pos(t, dt) : t + dt;
   s.nx = bx ? nt : nx;
   s.ny = by ? nt : ny;
   s.nz = bz ? nt : nz;

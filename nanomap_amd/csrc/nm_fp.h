@@ -109,6 +109,17 @@ NM_HD T add_down(T a, T b) {
   return err < T(0) ? next_down(s) : s;
 }
 
+// Round-down sum of a >= 0 and b > 0 (the DDA's next-crossing update),
+// branch-free: Fast2Sum on (max, min), then one ulp down if the error is
+// negative.  The sum is positive, so the step down is an integer decrement.
+NM_HD double add_down_pos(double a, double b) {
+  const double s = a + b;
+  const double big = a > b ? a : b;
+  const double small = a > b ? b : a;
+  const double err = small - (s - big);
+  return err < 0.0 ? double_of(bits_of(s) - 1) : s;
+}
+
 template <class T>
 NM_HD T mul_down(T a, T b) {
   const T p = a * b;

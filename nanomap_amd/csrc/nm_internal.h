@@ -42,7 +42,9 @@ struct MapView {
   float* values;                  // [leaf_cap * 512]
   uint32_t* masks;                // [leaf_cap * 16]
   int32_t* origins;               // [leaf_cap * 3]
-  int32_t* leaf_count;            // device counter
+  int32_t* leaf_count;            // device counters: [0] pool slots used,
+                                  // [1] holes (slots whose origin x is
+                                  // INT_MIN: reserved, never inserted)
   int64_t hash_cap;               // power of two
   int32_t leaf_cap;
   float clamp_min, clamp_max, hit_thr, miss_thr;
@@ -79,8 +81,10 @@ struct CastParams {
 // Device workspace of one context (one sensor stream).
 struct CtxBuffers {
   int32_t* leaf_slot;   // [cube_cap]  -1 free, -2 discovered, >= 0 slot
+                        //             (| kSlotHasHits once hits are sorted)
   int32_t* slot_leaf;   // [tile_cap]  cube index of each slot
   int32_t* map_idx;     // [tile_cap]  map leaf of each slot, -1 if new
+  int32_t* map_new;     // [tile_cap]  pool index reserved for a new leaf
   uint32_t* tiles;      // [tile_cap * 512 * 2]  (miss, hit) per voxel
   uint32_t* hit_mask;   // [tile_cap * 16]  voxels holding a hit
   uint32_t* hit_start;  // [tile_cap * 512] first sorted hit of a hit voxel
@@ -100,6 +104,9 @@ struct CtxBuffers {
 };
 
 constexpr int kScanChunk = 4096;
+// Bit of a cube-table entry marking a leaf that holds hit voxels (set after
+// the hit sort, cleared with the entry by the update).
+constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
 

@@ -176,6 +176,28 @@ def test_run_twice_is_identical(gpu):
         assert np.array_equal(a, b)
 
 
+def test_unchanged_leaves_are_not_inserted(gpu):
+    """A frame whose every temp value is zero (log-odds hit = miss = 0)
+    changes nothing (Handler.cpp:116-118 writes only nonzero temp values):
+    the leaves it reserved in the pool stay holes, invisible to leaf_count
+    and download, and later frames over the same leaves still match."""
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    pos, quat, cloud = frames[0]
+    z = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 10.0, 0.0, 0.0, RD)
+    _run_frame(nm, ctx, gmap, omap, z, cloud)
+    assert gmap.leaf_count() == 0
+    assert ctx.stats()["map_leaves"] == 0
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD)
+        _run_frame(nm, ctx, gmap, omap, f, cloud, check_counts=False)
+    _check_maps(gmap, omap)
+    assert ctx.stats()["map_leaves"] == gmap.leaf_count() > 0
+
+
 @pytest.mark.parametrize("rounding", [RD, RN])
 def test_voxel_filter_bit_exact(gpu, rounding):
     nm = gpu
