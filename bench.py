@@ -38,7 +38,25 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=30,
                     help="frames of the CPU baseline sample (0 = skip)")
     ap.add_argument("--rounding", choices=["down", "nearest"], default="down")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the timed steps, run one fill and one "
+                         "elementwise copy of this many MiB (the known-size "
+                         "kernels that calibrate the HBM byte counters)")
     return ap.parse_args()
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of the stage's kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    counters' reading for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(stage)
+    return None if k is None else k.get("bytes_per_launch")
 
 
 def percentile(xs, q):
@@ -107,6 +125,15 @@ def main():
     gmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
     print("setup %.1f s" % (time.perf_counter() - t_setup), file=sys.stderr,
           flush=True)
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib << 18
+        a = torch.ones(n, device="cuda:0")  # fill: writes 4n bytes
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)  # copy: reads 4n, writes 4n bytes
+        torch.cuda.synchronize()
+        del a, b
+        print("copy probe %d MiB" % args.copy_probe_mib, file=sys.stderr,
+              flush=True)
 
     def submit(k):
         ptr = dev[k].data_ptr()
@@ -222,7 +249,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom),
             "alg_bytes_per_launch": alg_bytes[dom],
         },
         "roofline_frame": {

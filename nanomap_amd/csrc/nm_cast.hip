@@ -557,7 +557,59 @@ __device__ bool map_insert(const MapView& m, int x, int y, int z, int idx) {
 }
 
 template <int M>
-__global__ __launch_bounds__(512) void k_update(
+__device__ __noinline__ float fold_call(float v, float a, uint32_t k) {
+  return fold<M>(v, a, k);
+}
+
+// fold<M> with its common short case (a few steps) inline and the rest out
+// of line, for call sites that are unrolled.
+template <int M>
+__device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
+  if (k <= 2) {
+    if (k > 0) v = add<M>(v, a);
+    if (k > 1) v = add<M>(v, a);
+    return v;
+  }
+  return fold_call<M>(v, a, k);
+}
+
+// Pixel-order sum of one voxel's hit sequence, computed by a whole
+// wavefront: gap[g] misses precede hit g (g < j).  The wavefront loads the
+// gaps 64 at a time (the next chunk is in flight while one is summed); a
+// run of consecutive zero gaps is a run of hits, summed by one fold.  Every
+// lane returns the same value; *used = total of the gaps.
+template <int M>
+__device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
+                                   uint32_t j, float miss, float hit,
+                                   uint32_t* used_out) {
+  const uint32_t lane = threadIdx.x & 63;
+  float acc = 0.0f;
+  uint32_t used = 0;
+  uint32_t cur = lane < j ? gap[lane] : 0u;
+  for (uint32_t base = 0; base < j; base += 64) {
+    const uint32_t n = j - base < 64u ? j - base : 64u;
+    const uint32_t nxt = base + 64 + lane < j ? gap[base + 64 + lane] : 0u;
+    uint64_t nzm = __ballot(cur != 0u);
+    uint32_t pos = 0;
+    while (nzm) {
+      const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
+      nzm &= nzm - 1;
+      acc = fold<M>(acc, hit, p - pos);
+      const uint32_t gp = (uint32_t)__shfl((int)cur, (int)p, 64);
+      acc = fold<M>(acc, miss, gp);
+      acc = add<M>(acc, hit);
+      used += gp;
+      pos = p + 1;
+    }
+    acc = fold<M>(acc, hit, n - pos);
+    cur = nxt;
+  }
+  *used_out = used;
+  return acc;
+}
+
+template <int M>
+__global__ __launch_bounds__(512, 4) void k_update(
     Cube cube, int32_t* __restrict__ leaf_slot,
     const int32_t* __restrict__ slot_leaf, uint32_t* __restrict__ tiles,
     uint32_t* __restrict__ hit_mask, const uint32_t* __restrict__ hit_start,
@@ -567,6 +619,9 @@ __global__ __launch_bounds__(512) void k_update(
     uint64_t* __restrict__ part) {
   __shared__ uint64_t s_part[8][3];
   __shared__ int s_new;
+  __shared__ int s_wcount[8];
+  __shared__ int s_list[512];
+  __shared__ float s_tv[512];
   const int li = threadIdx.x;
   const int lane = li & 63, wave = li >> 6;
   const int nleaves = min(cnt->n_leaves, tile_cap);
@@ -591,21 +646,64 @@ __global__ __launch_bounds__(512) void k_update(
       old_w = m.masks[(size_t)known * kMaskWords + (li >> 5)];
     }
     float tv = 0.0f;
-    if (touched) {
+    const bool heavy = j > kHeavyHits;
+    if (touched && !heavy) {
       if (j == 0) {
         tv = fold<M>(0.0f, miss, k);
       } else {
+        // gaps are loaded 4 at a time so the loads overlap
         const uint32_t s0 = hit_start[cell];
         uint32_t used = 0;
         float acc = 0.0f;
-        for (uint32_t g = 0; g < j; ++g) {
-          const uint32_t gm = gap[s0 + g];
-          acc = fold<M>(acc, miss, gm);
-          used += gm;
-          acc = add<M>(acc, hit);
+        for (uint32_t g = 0; g < j; g += 4) {
+          uint32_t gm[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (g + u < j) {
+              if (gm[u] != 0u) {
+                acc = fold_small<M>(acc, miss, gm[u]);
+                used += gm[u];
+              }
+              acc = add<M>(acc, hit);
+            }
+          }
         }
         tv = fold<M>(acc, miss, k - used);
       }
+    }
+    // Voxels with many hits (a surface close to the sensor) are summed by a
+    // whole wavefront each, so one long serial sequence does not hold up
+    // the leaf.
+    if (__syncthreads_or(heavy)) {
+      const uint64_t hb = __ballot(heavy);
+      if (lane == 0) s_wcount[wave] = (int)__popcll(hb);
+      __syncthreads();
+      int before = 0, nheavy = 0;
+      for (int w = 0; w < 8; ++w) {
+        before += w < wave ? s_wcount[w] : 0;
+        nheavy += s_wcount[w];
+      }
+      if (heavy) {
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        s_list[before + (int)__popcll(hb & lt)] = li;
+      }
+      __syncthreads();
+      for (int q = wave; q < nheavy; q += 8) {
+        const int v = s_list[q];
+        const size_t cv = (size_t)slot * 512 + v;
+        const uint32_t kv = tiles[2 * cv];
+        const uint32_t jv = tiles[2 * cv + 1];
+        uint32_t used = 0;
+        const float acc =
+            wave_hit_sequence<M>(gap + hit_start[cv], jv, miss, hit, &used);
+        const float t = fold<M>(acc, miss, kv - used);
+        if (lane == 0) s_tv[v] = t;
+      }
+      __syncthreads();
+      if (heavy) tv = s_tv[li];
     }
     const bool nz = touched && tv != 0.0f;
     const int any = __syncthreads_or(nz);

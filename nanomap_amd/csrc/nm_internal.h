@@ -109,6 +109,8 @@ constexpr int kScanChunk = 4096;
 constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
+// Voxels with more hits than this in one frame are summed by a wavefront.
+constexpr uint32_t kHeavyHits = 256;
 
 // ---- launchers (nm_cast.hip, nm_radix.hip, nm_filter.hip) -------------
 hipError_t launch_reset_counters(FrameCounters* c, hipStream_t s);
