@@ -33,18 +33,22 @@ struct nm_ctx {
   int cube_side_max = 0;
   int64_t cube_cap = 0;
   int tile_cap = 0;
+  int cu_count = 256;
   size_t staging_bytes = 0;
-  nm::CtxBuffers b{};
+  nm::Workspace b{};
   nm::FrameCounters* host_counters = nullptr;
+  nm::JobStats* host_stats = nullptr;
   // state of the last nm_cast
   bool pending = false;
   nm_frame frame{};
   nm::Cube cube{};
+  nm::Job job{};
+  nm::Pipeline pipe{};
   int n = 0;
   int sorted_buf = 0;
   hipStream_t stream = nullptr;
   nm_map* last_map = nullptr;
-  bool last_update_valid = false;  // part_update holds the last frame's sums
+  bool last_update_valid = false;  // slot_stats hold the last frame's sums
   // stage timing
   bool timing = false;
   struct Mark {
@@ -163,16 +167,9 @@ void free_ctx(nm_ctx* c) {
     (void)hipEventDestroy(m.b);
   }
   for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
-  nm::CtxBuffers& b = c->b;
-  void* ptrs[] = {b.leaf_slot,  b.slot_leaf, b.map_idx, b.map_new,  b.tiles,      b.hit_mask, b.hit_start,
-                  b.ray_hit,    b.keys[0],    b.keys[1],    b.vals[0],
-                  b.vals[1],    b.gap,        b.radix_hist, b.scan_blocks,
-                  b.counters,   b.part_discover, b.part_update,
-                  b.cloud,      b.filter_rays, b.filter_tmp,
-                  b.flags};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+  nm::workspace_free(&c->b);
   if (c->host_counters) (void)hipHostFree(c->host_counters);
+  if (c->host_stats) (void)hipHostFree(c->host_stats);
   delete c;
 }
 
@@ -213,50 +210,25 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   }
   c->tile_cap = cfg->tile_capacity > 0 ? cfg->tile_capacity : (int)c->cube_cap;
   if ((int64_t)c->tile_cap > c->cube_cap) c->tile_cap = (int)c->cube_cap;
-  nm::CtxBuffers& b = c->b;
   const size_t P = (size_t)cfg->max_points;
-  const size_t scan_blocks =
-      (size_t)std::max<int64_t>(c->cube_cap, (int64_t)P) / nm::kScanChunk + 2;
+  nm::WorkspaceDims d;
+  d.cube_cap = c->cube_cap;
+  d.tile_cap = c->tile_cap;
+  d.ray_cap = (int64_t)P;
+  // 16x16 items: ceil(W/16)*ceil(H/16) <= P/256 + (W+H)/16 + 1, W+H <= P+1
+  d.item_cap = (int)(P / 16 + P / 256 + 64);
+  d.max_jobs = 1;
+  d.host_clouds = true;
   c->staging_bytes = P * 16;
-  hipError_t e = hipSuccess;
-  auto A = [&](hipError_t r) {
-    if (e == hipSuccess) e = r;
-  };
-  A(dalloc(&b.leaf_slot, (size_t)c->cube_cap));
-  A(dalloc(&b.slot_leaf, (size_t)c->tile_cap));
-  A(dalloc(&b.map_idx, (size_t)c->tile_cap));
-  A(dalloc(&b.map_new, (size_t)c->tile_cap));
-  A(dalloc(&b.tiles, (size_t)c->tile_cap * 1024));
-  A(dalloc(&b.hit_mask, (size_t)c->tile_cap * nm::kMaskWords));
-  A(dalloc(&b.hit_start, (size_t)c->tile_cap * 512));
-  A(dalloc(&b.ray_hit, P));
-  A(dalloc(&b.keys[0], P));
-  A(dalloc(&b.keys[1], P));
-  A(dalloc(&b.vals[0], P));
-  A(dalloc(&b.vals[1], P));
-  A(dalloc(&b.gap, P));
-  A(dalloc(&b.radix_hist, (size_t)nm::radix_hist_words((int)P)));
-  A(dalloc(&b.scan_blocks, scan_blocks));
-  A(dalloc(&b.counters, 1));
-  // discover grid: at most ceil(W/16)*ceil(H/16) <= P/256 + (W+H)/16 + 1
-  // <= P/16 + P/256 + 2 blocks (W + H <= W*H + 1), or P/256 + 1
-  A(dalloc(&b.part_discover, (P / 16 + P / 256 + 64) * 4));
-  A(dalloc(&b.part_update, (size_t)nm::kUpdateGrid * 4));
-  A(dalloc(&b.cloud, c->staging_bytes));
-  A(dalloc(&b.filter_rays, P * 4));
-  A(dalloc(&b.filter_tmp, P * 4));
-  A(dalloc(&b.flags, P));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    c->cu_count = prop.multiProcessorCount;
+  hipError_t e = nm::workspace_alloc(d, &c->b);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&c->host_counters, sizeof(nm::FrameCounters));
   if (e == hipSuccess)
-    e = hipMemset(b.leaf_slot, 0xff, (size_t)c->cube_cap * sizeof(int32_t));
-  if (e == hipSuccess)
-    e = hipMemset(b.tiles, 0, (size_t)c->tile_cap * 1024 * sizeof(uint32_t));
-  if (e == hipSuccess)
-    e = hipMemset(b.hit_mask, 0,
-                  (size_t)c->tile_cap * nm::kMaskWords * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemset(b.counters, 0, sizeof(nm::FrameCounters));
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+    e = hipHostMalloc((void**)&c->host_stats, sizeof(nm::JobStats));
   if (e != hipSuccess) {
     free_ctx(c);
     return fail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
@@ -449,7 +421,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     pts = ctx->b.cloud;
   }
 
-  nm::CastParams p{};
+  nm::Job p{};
   memcpy(p.R, frame->rotation, sizeof(p.R));
   memcpy(p.eye, frame->eye, sizeof(p.eye));
   p.res = frame->grid_res;
@@ -467,42 +439,56 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
              frame->filter == NM_FILTER_NONE)
                 ? 1
                 : 0;
-  p.use_rays = 0;
-  p.rays4 = nullptr;
-  p.n_dev = nullptr;
+  p.layout = nm::kLayout16x16;
+  p.use_rays = frame->filter == NM_FILTER_VOXEL ? 1 : 0;
+  p.rays4 = p.use_rays ? ctx->b.filter_rays : nullptr;
+  p.n_dev = p.use_rays ? &ctx->b.counters->n_filter_rays : nullptr;
+  p.map = 0;
+  p.cube_off = 0;
+  p.ray_off = 0;
+  p.item_off = 0;
+  p.nitems = nm::job_items(p, p.layout);
 
-  NM_HIP(nm::launch_reset_counters(ctx->b.counters, s));
+  nm::Pipeline& pl = ctx->pipe;
+  pl = nm::Pipeline{};
+  pl.host_jobs = &ctx->job;
+  pl.njobs = 1;
+  pl.cube_vol = (int64_t)cube.side * cube.side * cube.side;
+  pl.n_rays = n_points;
+  pl.nitems = p.nitems;
+  pl.tile_cap = ctx->tile_cap;
+  pl.sort_bits = sort_bits_for((int64_t)ctx->tile_cap * 512);
+  pl.rounding = frame->rounding;
+  pl.cu_count = ctx->cu_count;
+  ctx->job = p;
+
+  NM_HIP(nm::launch_begin(ctx->b, &p, s));
   if (frame->filter == NM_FILTER_VOXEL) {
     StageTimer t(ctx, NM_STAGE_FILTER, s);
     float origin[3];
     for (int a = 0; a < 3; ++a)
       origin[a] = (float)(frame->eye[a] - floor(frame->eye[a]));
     NM_HIP(nm::launch_voxel_filter(p, origin, ctx->b, s));
-    p.use_rays = 1;
-    p.rays4 = ctx->b.filter_rays;
-    p.n_dev = &ctx->b.counters->n_filter_rays;
   }
-  const int64_t vol = (int64_t)cube.side * cube.side * cube.side;
   {
     StageTimer t(ctx, NM_STAGE_DISCOVER, s);
-    NM_HIP(nm::launch_discover(frame->rounding, p, ctx->b, s));
+    NM_HIP(nm::launch_discover(pl, ctx->b, s));
   }
   {
     StageTimer t(ctx, NM_STAGE_COMPACT, s);
-    NM_HIP(nm::launch_compact_leaves(p, ctx->b, vol, ctx->tile_cap, s));
+    NM_HIP(nm::launch_compact_leaves(pl, ctx->b, s));
   }
   int sorted = 0;
   {
     StageTimer t(ctx, NM_STAGE_HITSORT, s);
-    NM_HIP(nm::launch_hit_keys(p, ctx->b, s));
-    const int bits = sort_bits_for((int64_t)ctx->tile_cap * 512);
-    NM_HIP(nm::radix_sort_pairs(ctx->b.keys, ctx->b.vals, n_points, bits,
-                                ctx->b.radix_hist, &sorted, s));
-    NM_HIP(nm::launch_hit_segments(ctx->b, n_points, sorted, s));
+    NM_HIP(nm::launch_hit_keys(pl, ctx->b, s));
+    NM_HIP(nm::radix_sort_pairs(ctx->b.keys, ctx->b.vals, n_points,
+                                pl.sort_bits, ctx->b.radix_hist, &sorted, s));
+    NM_HIP(nm::launch_hit_segments(pl, ctx->b, sorted, s));
   }
   {
     StageTimer t(ctx, NM_STAGE_ACCUMULATE, s);
-    NM_HIP(nm::launch_accumulate(frame->rounding, p, ctx->b, sorted, s));
+    NM_HIP(nm::launch_accumulate(pl, ctx->b, sorted, s));
   }
   ctx->pending = true;
   ctx->last_update_valid = false;
@@ -523,9 +509,7 @@ int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   {
     StageTimer t(ctx, NM_STAGE_UPDATE, s);
-    NM_HIP(nm::launch_update(ctx->frame.rounding, ctx->cube, ctx->b, map->v,
-                             ctx->frame.logodds_hit, ctx->frame.logodds_miss,
-                             ctx->sorted_buf, ctx->tile_cap, s));
+    NM_HIP(nm::launch_update(ctx->pipe, ctx->b, nullptr, map->v, s));
   }
   ctx->pending = false;
   ctx->last_update_valid = true;
@@ -551,24 +535,23 @@ int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out) {
                         ctx->stream));
   NM_HIP(hipStreamSynchronize(ctx->stream));
   const nm::FrameCounters& c = *ctx->host_counters;
-  uint64_t upd[3] = {0, 0, 0};
-  if (ctx->last_update_valid) {
-    std::vector<uint64_t> part((size_t)nm::kUpdateGrid * 4);
-    NM_HIP(hipMemcpy(part.data(), ctx->b.part_update,
-                     part.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    for (int b = 0; b < nm::kUpdateGrid; ++b)
-      for (int q = 0; q < 3; ++q) upd[q] += part[(size_t)b * 4 + q];
-  }
   memset(out, 0, sizeof(*out));
   out->points = ctx->n;
   out->rays = (int64_t)c.rays;
   out->visits = (int64_t)c.visits;
-  out->voxels = (int64_t)upd[0];
   out->leaves = c.n_leaves;
-  out->hit_voxels = (int64_t)upd[1];
-  out->mixed_voxels = (int64_t)upd[2];
   out->errors = c.errors;
   out->overflow = c.overflow;
+  if (ctx->last_update_valid) {
+    NM_HIP(nm::launch_job_stats(ctx->pipe, ctx->b, ctx->stream));
+    NM_HIP(hipMemcpyAsync(ctx->host_stats, ctx->b.job_stats,
+                          sizeof(nm::JobStats), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    NM_HIP(hipStreamSynchronize(ctx->stream));
+    out->voxels = (int64_t)ctx->host_stats->voxels;
+    out->hit_voxels = (int64_t)ctx->host_stats->hit_voxels;
+    out->mixed_voxels = (int64_t)ctx->host_stats->mixed_voxels;
+  }
   if (ctx->last_map) {
     int32_t n[2] = {0, 0};
     NM_HIP(hipMemcpy(n, ctx->last_map->v.leaf_count, sizeof(n),

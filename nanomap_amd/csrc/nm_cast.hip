@@ -4,30 +4,39 @@
 // and the host fold into the map (Handler::voxelUpdateFromFrustumBuffer,
 // src/nanomap/handler/Handler.cpp:134), with the semantics of the
 // reference CPU path (Handler::populateTempGrid/integrateTempGrid,
-// Handler.cpp:28-122).  Per frame:
+// Handler.cpp:28-122).  Every kernel reads its frames from a device table
+// of jobs (one frame of one sensor each), so one sequence of launches
+// serves one frame or a batch of many sensors with their own poses, ranges,
+// voxel sizes and maps.  Per sequence:
 //
 //  1 discover   one lane per point: ray set-up + voxel DDA (nm_dda.h);
-//               marks every 8^3 leaf the ray enters in a dense cube table
-//               around the eye, records the cell of the ray's hit voxel.
-//  2 compact    prefix sum over the cube table -> dense tile slots, in
-//               ascending cube order (deterministic).
-//  3 hit sort   (tile cell, pixel) pairs of all hits, stable radix sort:
+//               marks every 8^3 leaf the ray enters in the job's dense
+//               cube table around the eye, records the cell of the ray's
+//               hit voxel.
+//  2 compact    prefix sum over all jobs' cube tables -> dense tile slots,
+//               in ascending (job, cube) order (deterministic).
+//  3 hit sort   (tile cell, point) pairs of all hits, stable radix sort:
 //               every hit voxel's hits in ascending pixel order.
 //  4 segments   per hit voxel: hit count into its tile, hit bit in the
 //               tile's 512-bit mask, zeroed gap counters.
 //  5 accumulate one lane per point again: integer miss counts into the
-//               tiles (atomics, merged across the wavefront when every
-//               lane hits the same voxel).  A miss in a voxel that also
-//               holds hits increments the gap counter of the first later
-//               hit (binary search in the sorted hits), so the order of
-//               hits and misses in pixel order is known exactly.
+//               tiles (LDS-aggregated per work item).  A miss in a voxel
+//               that also holds hits increments the gap counter of the
+//               first later hit (binary search in the sorted hits), so the
+//               order of hits and misses in pixel order is known exactly.
 //  6 update     one workgroup per tile: the temp value of each voxel is
 //               the FP32 sum of its events in pixel order, rebuilt from
 //               the counts with nm::fold (exact, any rounding mode); then
 //               map += temp, clamp, activity thresholds; tiles cleared.
 //
-// Counts are integers, so the result does not depend on the order in
-// which lanes run: the map is bit-identical to the sequential CPU fold.
+// Counts are integers, so the result depends neither on the order in which
+// lanes run nor on how the work items are shaped and scheduled: each job's
+// map is bit-identical to the sequential CPU fold of that frame alone.
+//
+// Work items of the cast kernels are 256 rays: a pixel tile of an image
+// cloud (layouts in nm_internal.h) or 256 consecutive points otherwise.
+// They are taken one per workgroup, or drawn by persistent workgroups from
+// a device queue (job order or costliest first).
 #include <climits>
 
 #include "nm_dda.h"
@@ -55,8 +64,7 @@ __device__ __forceinline__ int cube_index(const Cube& c, int x, int y,
 }
 
 template <int M>
-__device__ __forceinline__ bool ray_of(const CastParams& p, int i,
-                                       RayDir& r) {
+__device__ __forceinline__ bool ray_of(const Job& p, int i, RayDir& r) {
   if (p.use_rays) {
     if (i >= *p.n_dev) return false;
     const float4 q = reinterpret_cast<const float4*>(p.rays4)[i];
@@ -78,36 +86,84 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   return v;
 }
 
-// Point handled by thread t of block b.  Image clouds are walked in 16x16
-// pixel tiles (one 8x8 sub-tile per wavefront) so the rays of a wavefront
-// stay close together: less divergence in the DDA, more shared voxels for
-// the LDS aggregation.  Other clouds are walked linearly.
-__device__ __forceinline__ int point_of(const CastParams& p, int b, int t) {
-  if (p.tiled) {
-    const int tiles_x = (p.width + 15) >> 4;
-    const int sub = t >> 6, lane = t & 63;
-    const int x = (b % tiles_x) * 16 + (sub & 1) * 8 + (lane & 7);
-    const int y = (b / tiles_x) * 16 + (sub >> 1) * 8 + (lane >> 3);
-    if (x >= p.width || y >= p.height) return -1;
-    return y * p.width + x;
+__host__ __device__ inline void layout_dims(int layout, int& tw, int& th,
+                                            int& ww) {
+  switch (layout) {
+    case kLayout32x8: tw = 32; th = 8; ww = 8; break;
+    case kLayout64x4: tw = 64; th = 4; ww = 16; break;
+    case kLayout8x32: tw = 8; th = 32; ww = 8; break;
+    default: tw = 16; th = 16; ww = 8; break;
   }
-  const int i = b * kCastThreads + t;
-  return i < p.n ? i : -1;
 }
 
-__host__ __device__ inline int cast_grid(const CastParams& p) {
-  if (p.tiled) return ((p.width + 15) >> 4) * ((p.height + 15) >> 4);
-  return (p.n + kCastThreads - 1) / kCastThreads;
+// Point of thread `tid` in work item `t` of job J, or -1.  Image clouds
+// are walked in pixel tiles so the rays of a wavefront stay close together:
+// less divergence in the DDA, more shared voxels for the LDS aggregation.
+__device__ __forceinline__ int point_of(const Job& J, int t, int tid) {
+  if (J.tiled) {
+    int tw, th, ww;
+    layout_dims(J.layout, tw, th, ww);
+    const int w = tid >> 6, lane = tid & 63;
+    int sx, sy;
+    if (J.layout == kLayout16x16) {
+      sx = (w & 1) * 8;
+      sy = (w >> 1) * 8;
+    } else if (J.layout == kLayout8x32) {
+      sx = 0;
+      sy = w * 8;
+    } else {
+      sx = w * ww;
+      sy = 0;
+    }
+    const int tiles_x = (J.width + tw - 1) / tw;
+    const int x = (t % tiles_x) * tw + sx + lane % ww;
+    const int y = (t / tiles_x) * th + sy + lane / ww;
+    if (x >= J.width || y >= J.height) return -1;
+    return y * J.width + x;
+  }
+  const int i = t * kCastThreads + tid;
+  return i < J.n ? i : -1;
 }
 
-__global__ void k_reset_counters(FrameCounters* c) {
-  if (threadIdx.x == 0) *c = FrameCounters{};
+// Work items of a launch: one per workgroup (head == null) or drawn from
+// the queue at *head by persistent workgroups, in `order` if given.
+struct ItemQueue {
+  const int32_t* item_job;  // job of each item (null: job 0)
+  const int32_t* order;
+  int32_t* head;
+  int nitems;
+};
+
+// Next item of this workgroup, or -1.  `round` counts the calls.
+__device__ __forceinline__ int take_item(const ItemQueue& q, int& round) {
+  if (!q.head) {
+    const int it = (int)blockIdx.x;
+    return (round++ == 0 && it < q.nitems) ? it : -1;
+  }
+  __shared__ int s_item;
+  __syncthreads();  // every lane is done with the previous item
+  if (threadIdx.x == 0) s_item = atomicAdd(q.head, 1);
+  __syncthreads();
+  const int it = __builtin_amdgcn_readfirstlane(s_item);
+  if (it >= q.nitems) return -1;
+  return q.order ? __builtin_amdgcn_readfirstlane(q.order[it]) : it;
 }
 
-// Per-block partial sums of the frame counters, written with plain stores
-// and reduced by one workgroup later: a global atomic per wave on a handful
-// of addresses serializes at the memory side and cost more than the
-// traversal itself.
+__device__ __forceinline__ int job_of_item(const ItemQueue& q, int item) {
+  return q.item_job ? __builtin_amdgcn_readfirstlane(q.item_job[item]) : 0;
+}
+
+__global__ void k_begin(Job j0, int has_job0, Job* __restrict__ jobs,
+                        FrameCounters* __restrict__ c) {
+  if (threadIdx.x == 0) {
+    *c = FrameCounters{};
+    if (has_job0) jobs[0] = j0;
+  }
+}
+
+// Per-item partial sums of the counters, written with plain stores and
+// reduced later: global atomics on a handful of addresses serialize at the
+// memory side and cost more than the traversal itself.
 __device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
                                                const uint64_t* v, int n) {
   __shared__ uint64_t s_part[4][4];
@@ -118,56 +174,63 @@ __device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
   }
   __syncthreads();
   if (threadIdx.x < n)
-    out[(size_t)blockIdx.x * 4 + threadIdx.x] =
-        s_part[0][threadIdx.x] + s_part[1][threadIdx.x] +
-        s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
+    out[threadIdx.x] = s_part[0][threadIdx.x] + s_part[1][threadIdx.x] +
+                       s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
 }
 
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
-    CastParams p, int32_t* __restrict__ leaf_slot,
+    const Job* __restrict__ jobs, ItemQueue q, int32_t* __restrict__ leaf_slot,
     uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part) {
-  const int i = point_of(p, blockIdx.x, threadIdx.x);
-  uint32_t hit_cell = kInvalid;
-  uint32_t visits = 0;
-  int valid = 0, errors = 0;
-  RayDir r;
-  if (i >= 0 && ray_of<M>(p, i, r)) {
-    valid = 1;
-    Dda s;
-    dda_init<M>(p.eye, r, s);
-    int c = cube_index(p.cube, s.vx, s.vy, s.vz);
-    if (c < 0)
-      ++errors;
-    else
-      leaf_slot[c] = -2;
-    int px = s.vx, py = s.vy, pz = s.vz;
-    while (dda_step<M>(s, r.L)) {
-      ++visits;  // miss on (px, py, pz)
-      if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7) {
-        c = cube_index(p.cube, s.vx, s.vy, s.vz);
-        if (c < 0)
-          ++errors;
-        else
-          leaf_slot[c] = -2;
+  int round = 0;
+  for (;;) {
+    const int item = take_item(q, round);
+    if (item < 0) break;
+    const Job J = jobs[job_of_item(q, item)];
+    int32_t* __restrict__ ls = leaf_slot + J.cube_off;
+    const int i = point_of(J, item - J.item_off, threadIdx.x);
+    uint32_t hit_cell = kInvalid;
+    uint32_t visits = 0;
+    int valid = 0, errors = 0;
+    RayDir r;
+    if (i >= 0 && ray_of<M>(J, i, r)) {
+      valid = 1;
+      Dda s;
+      dda_init<M>(J.eye, r, s);
+      int c = cube_index(J.cube, s.vx, s.vy, s.vz);
+      if (c < 0)
+        ++errors;
+      else
+        ls[c] = -2;
+      int px = s.vx, py = s.vy, pz = s.vz;
+      while (dda_step<M>(s, r.L)) {
+        ++visits;  // miss on (px, py, pz)
+        if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7) {
+          c = cube_index(J.cube, s.vx, s.vy, s.vz);
+          if (c < 0)
+            ++errors;
+          else
+            ls[c] = -2;
+        }
+        px = s.vx;
+        py = s.vy;
+        pz = s.vz;
       }
-      px = s.vx;
-      py = s.vy;
-      pz = s.vz;
+      if (s.t < (double)J.max_time) {
+        ++visits;
+        const int ce = cube_index(J.cube, px, py, pz);
+        if (ce >= 0)
+          hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(px, py, pz);
+      }
     }
-    if (s.t < (double)p.max_time) {
-      ++visits;
-      const int ce = cube_index(p.cube, px, py, pz);
-      if (ce >= 0) hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(px, py, pz);
-    }
+    if (i >= 0) ray_hit[(size_t)J.ray_off + i] = hit_cell;
+    const uint64_t v[4] = {visits, (uint64_t)valid,
+                           (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
+    block_partials(part + (size_t)item * 4, v, 4);
   }
-  if (i >= 0) ray_hit[i] = hit_cell;
-  const uint64_t v[4] = {visits, (uint64_t)valid,
-                         (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
-  block_partials(part, v, 4);
 }
 
-// ---- leaf compaction: exclusive scan over the cube table --------------
+// ---- leaf compaction: exclusive scan over the cube tables --------------
 __global__ __launch_bounds__(256) void k_scan_count(
     const int32_t* __restrict__ leaf_slot, int64_t vol,
     int32_t* __restrict__ block_counts) {
@@ -185,7 +248,8 @@ __global__ __launch_bounds__(256) void k_scan_count(
 }
 
 // One workgroup: exclusive scan of block_counts; total -> n_leaves.  Also
-// reduces the discover partials (visits, rays, hit rays, errors).
+// reduces the discover partials of all items (visits, rays, hit rays,
+// errors) into the sequence counters.
 __global__ __launch_bounds__(1024) void k_scan_blocks(
     int32_t* __restrict__ block_counts, int nblocks,
     const uint64_t* __restrict__ part, int npart,
@@ -244,11 +308,13 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(
 __global__ __launch_bounds__(256) void k_scan_assign(
     int32_t* __restrict__ leaf_slot, int64_t vol,
     const int32_t* __restrict__ block_counts,
-    int32_t* __restrict__ slot_leaf, int tile_cap,
+    const int32_t* __restrict__ scan_job, int32_t* __restrict__ slot_leaf,
+    int32_t* __restrict__ slot_job, int tile_cap,
     FrameCounters* __restrict__ cnt) {
   __shared__ int ws[4];
   const int64_t base = (int64_t)blockIdx.x * kScanChunk;
   int run = block_counts[blockIdx.x];
+  const int job = scan_job[blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (int j0 = 0; j0 < kScanChunk; j0 += 256) {
@@ -263,8 +329,9 @@ __global__ __launch_bounds__(256) void k_scan_assign(
       if (pre < tile_cap) {
         leaf_slot[c] = pre;
         slot_leaf[pre] = (int32_t)c;
+        slot_job[pre] = job;
       } else {
-        leaf_slot[c] = -1;  // dropped: the frame reports overflow
+        leaf_slot[c] = -1;  // dropped: the sequence reports overflow
         atomicAdd(&cnt->overflow, 1);
       }
     }
@@ -274,21 +341,26 @@ __global__ __launch_bounds__(256) void k_scan_assign(
 }
 
 // Sort keys of the hit events: tile cell (slot * 512 + local), value =
-// pixel index.  Rays without a hit get the largest key and sort last.
+// global point index.  Rays without a hit get the largest key, sort last.
 __global__ __launch_bounds__(256) void k_hit_keys(
-    int n, const uint32_t* __restrict__ ray_hit,
+    const Job* __restrict__ jobs, ItemQueue q,
+    const uint32_t* __restrict__ ray_hit,
     const int32_t* __restrict__ leaf_slot, uint32_t* __restrict__ keys,
     uint32_t* __restrict__ vals) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t h = ray_hit[i];
+  const int item = blockIdx.x;
+  if (item >= q.nitems) return;
+  const Job J = jobs[job_of_item(q, item)];
+  const int i = point_of(J, item - J.item_off, threadIdx.x);
+  if (i < 0) return;
+  const size_t g = (size_t)J.ray_off + i;
+  const uint32_t h = ray_hit[g];
   uint32_t key = kInvalid;
   if (h != kInvalid) {
-    const int slot = leaf_slot[h >> 9];
+    const int slot = leaf_slot[J.cube_off + (h >> 9)];
     if (slot >= 0) key = (uint32_t)slot * 512u + (h & 511u);
   }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  keys[g] = key;
+  vals[g] = (uint32_t)g;
 }
 
 __global__ __launch_bounds__(256) void k_hit_segments(
@@ -326,7 +398,7 @@ __global__ __launch_bounds__(256) void k_hit_segments(
 // count) per workgroup.  A 16x16 pixel tile touches a few thousand distinct
 // voxels for tens of thousands of visits, so the table turns almost every
 // visit into an LDS atomic and leaves one global atomic per distinct voxel
-// per workgroup, at flush.  A visit that finds no free slot within kProbe
+// per work item, at flush.  A visit that finds no free slot within kProbe
 // probes falls back to a global atomic.  Two unrolled probes measured
 // fastest: longer probe loops diverge across the wavefront and cost more
 // than the rare global fallback; 2048 or 8192 slots lose to 4096 (more
@@ -385,8 +457,9 @@ __device__ __forceinline__ int slot_of(const Cube& cube,
 
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_accumulate(
-    CastParams p, const int32_t* __restrict__ leaf_slot,
-    uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_mask,
+    const Job* __restrict__ jobs, ItemQueue q,
+    const int32_t* __restrict__ leaf_slot, uint32_t* __restrict__ tiles,
+    const uint32_t* __restrict__ hit_mask,
     const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
     FrameCounters* __restrict__ cnt) {
@@ -396,62 +469,75 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     s_key[k] = kInvalid;
     s_cnt[k] = 0;
   }
-  __syncthreads();
-  const int i = point_of(p, blockIdx.x, threadIdx.x);
-  RayDir r;
+  int round = 0;
   int errors = 0;
-  if (i >= 0 && ray_of<M>(p, i, r)) {
-    Dda s;
-    dda_init<M>(p.eye, r, s);
-    int px = s.vx, py = s.vy, pz = s.vz;
-    int entry = slot_of(p.cube, leaf_slot, px, py, pz);
-    while (dda_step<M>(s, r.L)) {
-      // miss on (px, py, pz)
-      if (entry < 0) {
-        ++errors;
-      } else {
-        const int slot = entry & ~kSlotHasHits;
-        const int li = local_of(px, py, pz);
-        const uint32_t cell = (uint32_t)slot * 512u + (uint32_t)li;
-        bool hit_voxel = false;
-        if (entry & kSlotHasHits) {
-          const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
-          hit_voxel = (mw >> (li & 31)) & 1u;
-        }
-        if (hit_voxel) {
-          // a voxel holding hits: count the miss directly and mark the
-          // gap before the first later hit of this voxel (its hits are
-          // sorted[s0, s0 + j) in ascending pixel order)
-          const uint32_t s0 = hit_start[cell];
-          const uint32_t j = tiles[2 * (size_t)cell + 1];
-          uint32_t lo = s0, hi = s0 + j;
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (svals[mid] < (uint32_t)i)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          if (lo < s0 + j) atomicAdd(&gap[lo], 1u);
-          atomicAdd(&tiles[2 * (size_t)cell], 1u);
+  for (;;) {
+    const int item = take_item(q, round);
+    if (item < 0) break;
+    __syncthreads();  // the table is clear
+    const Job J = jobs[job_of_item(q, item)];
+    const int32_t* __restrict__ ls = leaf_slot + J.cube_off;
+    const int i = point_of(J, item - J.item_off, threadIdx.x);
+    const uint32_t gi = (uint32_t)(J.ray_off + i);
+    RayDir r;
+    if (i >= 0 && ray_of<M>(J, i, r)) {
+      Dda s;
+      dda_init<M>(J.eye, r, s);
+      int px = s.vx, py = s.vy, pz = s.vz;
+      int entry = slot_of(J.cube, ls, px, py, pz);
+      while (dda_step<M>(s, r.L)) {
+        // miss on (px, py, pz)
+        if (entry < 0) {
+          ++errors;
         } else {
-          wave_lds_count(s_key, s_cnt, tiles, cell);
+          const int slot = entry & ~kSlotHasHits;
+          const int li = local_of(px, py, pz);
+          const uint32_t cell = (uint32_t)slot * 512u + (uint32_t)li;
+          bool hit_voxel = false;
+          if (entry & kSlotHasHits) {
+            const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
+            hit_voxel = (mw >> (li & 31)) & 1u;
+          }
+          if (hit_voxel) {
+            // a voxel holding hits: count the miss directly and mark the
+            // gap before the first later hit of this voxel (its hits are
+            // sorted[s0, s0 + j) in ascending point order)
+            const uint32_t s0 = hit_start[cell];
+            const uint32_t j = tiles[2 * (size_t)cell + 1];
+            uint32_t lo = s0, hi = s0 + j;
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (svals[mid] < gi)
+                lo = mid + 1;
+              else
+                hi = mid;
+            }
+            if (lo < s0 + j) atomicAdd(&gap[lo], 1u);
+            atomicAdd(&tiles[2 * (size_t)cell], 1u);
+          } else {
+            wave_lds_count(s_key, s_cnt, tiles, cell);
+          }
         }
+        if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
+          entry = slot_of(J.cube, ls, s.vx, s.vy, s.vz);
+        px = s.vx;
+        py = s.vy;
+        pz = s.vz;
       }
-      if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
-        entry = slot_of(p.cube, leaf_slot, s.vx, s.vy, s.vz);
-      px = s.vx;
-      py = s.vy;
-      pz = s.vz;
+    }
+    __syncthreads();
+    // flush the item's counts and leave the table empty for the next one
+    for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
+      const uint32_t key = s_key[k];
+      if (key != kInvalid) {
+        atomicAdd(&tiles[2 * (size_t)key], s_cnt[k]);
+        s_key[k] = kInvalid;
+        s_cnt[k] = 0;
+      }
     }
   }
   const uint64_t se = wave_sum((uint64_t)errors);
   if ((threadIdx.x & 63) == 0 && se) atomicAdd(&cnt->errors, (int)se);
-  __syncthreads();
-  for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
-    const uint32_t key = s_key[k];
-    if (key != kInvalid) atomicAdd(&tiles[2 * (size_t)key], s_cnt[k]);
-  }
 }
 
 // ---- map hash ----------------------------------------------------------
@@ -471,31 +557,47 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
   return k;
 }
 
+// Leaf origin (index space) of entry c of job J's cube table.
+__device__ __forceinline__ void slot_origin(const Job& J, int c, int& ox,
+                                            int& oy, int& oz) {
+  const int side = J.cube.side;
+  const int lz = c % side;
+  const int ly = (c / side) % side;
+  const int lx = c / (side * side);
+  ox = (lx + J.cube.cmin[0]) * 8;
+  oy = (ly + J.cube.cmin[1]) * 8;
+  oz = (lz + J.cube.cmin[2]) * 8;
+}
+
 // Map leaf of every discovered tile, one lane per tile so the probe
 // latencies overlap: map_idx = leaf index, or -1 if the leaf is new to the
-// map.  A new leaf gets a pool index reserved here, in tile order (one
-// atomic per wavefront), and its hash insertion point; the update inserts it
-// only if one of its voxels changes (Handler.cpp:116-118 touches the map
-// only for nonzero temp values), otherwise the reserved index is a hole.
+// map.  A new leaf gets a pool index reserved here, in slot order (one
+// atomic per wavefront and map), and its hash insertion point; the update
+// inserts it only if one of its voxels changes (Handler.cpp:116-118
+// touches the map only for nonzero temp values), otherwise the reserved
+// index is a hole.
 __global__ __launch_bounds__(256) void k_map_lookup(
-    Cube cube, const int32_t* __restrict__ slot_leaf, MapView m, int tile_cap,
-    FrameCounters* __restrict__ cnt, int32_t* __restrict__ map_idx,
-    int32_t* __restrict__ map_new) {
+    const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
+    const int32_t* __restrict__ slot_leaf, const MapView* __restrict__ maps,
+    MapView single, int tile_cap, FrameCounters* __restrict__ cnt,
+    int32_t* __restrict__ map_idx, int32_t* __restrict__ map_new) {
   const int nleaves = min(cnt->n_leaves, tile_cap);
-  const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
   const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (int base = blockIdx.x * 256; base < nleaves; base += gridDim.x * 256) {
     const int slot = base + threadIdx.x;
     int idx = -1;
     bool unknown = false;
+    int mapi = -1;
+    MapView m = single;
     if (slot < nleaves) {
-      const int c = slot_leaf[slot];
-      const int lz = c % cube.side;
-      const int ly = (c / cube.side) % cube.side;
-      const int lx = c / (cube.side * cube.side);
-      const unsigned long long key = pack_leaf((lx + cube.cmin[0]) * 8,
-                                               (ly + cube.cmin[1]) * 8,
-                                               (lz + cube.cmin[2]) * 8);
+      const Job& J = jobs[slot_job[slot]];
+      mapi = J.map;
+      if (maps) m = maps[mapi];
+      int ox, oy, oz;
+      slot_origin(J, slot_leaf[slot] - (int)J.cube_off, ox, oy, oz);
+      const unsigned long long key = pack_leaf(ox, oy, oz);
+      const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
       unsigned long long h = mix64(key) & mask;
       unknown = true;
       for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
@@ -509,20 +611,29 @@ __global__ __launch_bounds__(256) void k_map_lookup(
         h = (h + 1) & mask;
       }
     }
-    // reserve pool indices for the unknown leaves of this wavefront
-    const uint64_t want = __ballot(unknown);
+    // reserve pool indices for the unknown leaves, one atomic per map
+    // present in the wavefront
+    uint64_t want = __ballot(unknown);
     int first = 0;
-    if (want) {
-      if (lane == (int)__builtin_ctzll(want))
-        first = atomicAdd(m.leaf_count, (int)__popcll(want));
-      first = __shfl(first, (int)__builtin_ctzll(want), 64);
+    uint64_t mine = 0;
+    while (want) {
+      const int leader = (int)__builtin_ctzll(want);
+      const int lm = __shfl(mapi, leader, 64);
+      const uint64_t peers = want & __ballot(mapi == lm);
+      int f = 0;
+      if (lane == leader) f = atomicAdd(m.leaf_count, (int)__popcll(peers));
+      f = __shfl(f, leader, 64);
+      if (mapi == lm) {
+        first = f;
+        mine = peers;
+      }
+      want &= ~peers;
     }
     if (slot < nleaves) {
       map_idx[slot] = idx;
       int reserved = -1;
       if (unknown) {
-        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        reserved = first + (int)__popcll(want & lt);
+        reserved = first + (int)__popcll(mine & lt);
         if (reserved >= m.leaf_cap) {
           atomicAdd(&cnt->overflow, 1);
           reserved = -1;
@@ -610,14 +721,14 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
 
 template <int M>
 __global__ __launch_bounds__(512, 4) void k_update(
-    Cube cube, int32_t* __restrict__ leaf_slot,
-    const int32_t* __restrict__ slot_leaf, uint32_t* __restrict__ tiles,
-    uint32_t* __restrict__ hit_mask, const uint32_t* __restrict__ hit_start,
-    const uint32_t* __restrict__ gap, const int32_t* __restrict__ map_idx,
-    const int32_t* __restrict__ map_new, MapView m, float hit, float miss,
-    int tile_cap, FrameCounters* __restrict__ cnt,
-    uint64_t* __restrict__ part) {
-  __shared__ uint64_t s_part[8][3];
+    const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
+    int32_t* __restrict__ leaf_slot, const int32_t* __restrict__ slot_leaf,
+    uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
+    const uint32_t* __restrict__ hit_start, const uint32_t* __restrict__ gap,
+    const int32_t* __restrict__ map_idx, const int32_t* __restrict__ map_new,
+    const MapView* __restrict__ maps, MapView single, int tile_cap,
+    FrameCounters* __restrict__ cnt, uint32_t* __restrict__ slot_stats) {
+  __shared__ uint32_t s_part[8][3];
   __shared__ int s_new;
   __shared__ int s_wcount[8];
   __shared__ int s_list[512];
@@ -625,15 +736,14 @@ __global__ __launch_bounds__(512, 4) void k_update(
   const int li = threadIdx.x;
   const int lane = li & 63, wave = li >> 6;
   const int nleaves = min(cnt->n_leaves, tile_cap);
-  uint64_t n_vox = 0, n_hit = 0, n_mixed = 0;
   for (int slot = blockIdx.x; slot < nleaves; slot += gridDim.x) {
+    const int jb = __builtin_amdgcn_readfirstlane(slot_job[slot]);
+    const Job& J = jobs[jb];
+    const float hit = J.hit, miss = J.miss;
+    const MapView m = maps ? maps[J.map] : single;
     const int c = slot_leaf[slot];
-    const int lz = c % cube.side;
-    const int ly = (c / cube.side) % cube.side;
-    const int lx = c / (cube.side * cube.side);
-    const int ox = (lx + cube.cmin[0]) * 8;
-    const int oy = (ly + cube.cmin[1]) * 8;
-    const int oz = (lz + cube.cmin[2]) * 8;
+    int ox, oy, oz;
+    slot_origin(J, c - (int)J.cube_off, ox, oy, oz);
     const size_t cell = (size_t)slot * 512 + li;
     const uint32_t k = tiles[2 * cell];      // misses
     const uint32_t j = tiles[2 * cell + 1];  // hits
@@ -757,24 +867,82 @@ __global__ __launch_bounds__(512, 4) void k_update(
     }
     if (li < kMaskWords) hit_mask[(size_t)slot * kMaskWords + li] = 0;
     if (li == 0) leaf_slot[c] = -1;
-    n_vox += touched;
-    n_hit += (j != 0);
-    n_mixed += (j != 0 && k != 0);
+    // per-slot statistics: voxels touched, with hits, with hits and misses
+    const uint64_t bt = __ballot(touched);
+    const uint64_t bh = __ballot(j != 0);
+    const uint64_t bm = __ballot(j != 0 && k != 0);
+    if (lane == 0) {
+      s_part[wave][0] = (uint32_t)__popcll(bt);
+      s_part[wave][1] = (uint32_t)__popcll(bh);
+      s_part[wave][2] = (uint32_t)__popcll(bm);
+    }
+    __syncthreads();
+    if (li < 3) {
+      uint32_t t = 0;
+      for (int w = 0; w < 8; ++w) t += s_part[w][li];
+      slot_stats[(size_t)slot * 4 + li] = t;
+    }
     __syncthreads();
   }
-  n_vox = wave_sum(n_vox);
-  n_hit = wave_sum(n_hit);
-  n_mixed = wave_sum(n_mixed);
-  if (lane == 0) {
-    s_part[wave][0] = n_vox;
-    s_part[wave][1] = n_hit;
-    s_part[wave][2] = n_mixed;
+}
+
+// Without a map: clear the per-frame state of the last cast.
+__global__ __launch_bounds__(512) void k_clear_slots(
+    int32_t* __restrict__ leaf_slot, const int32_t* __restrict__ slot_leaf,
+    uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
+    const FrameCounters* __restrict__ cnt, int tile_cap) {
+  const int nleaves = min(cnt->n_leaves, tile_cap);
+  for (int slot = blockIdx.x; slot < nleaves; slot += gridDim.x) {
+    const size_t cell = (size_t)slot * 512 + threadIdx.x;
+    tiles[2 * cell] = 0;
+    tiles[2 * cell + 1] = 0;
+    if (threadIdx.x < kMaskWords)
+      hit_mask[(size_t)slot * kMaskWords + threadIdx.x] = 0;
+    if (threadIdx.x == 0) leaf_slot[slot_leaf[slot]] = -1;
+  }
+}
+
+// One workgroup per job: the job's statistics from the per-item discover
+// partials and the per-slot update statistics.
+__global__ __launch_bounds__(256) void k_job_stats(
+    const Job* __restrict__ jobs, int njobs,
+    const uint64_t* __restrict__ part, const int32_t* __restrict__ scan_blocks,
+    const uint32_t* __restrict__ slot_stats,
+    const FrameCounters* __restrict__ cnt, int tile_cap,
+    JobStats* __restrict__ out) {
+  __shared__ unsigned long long s[7];
+  const int jb = blockIdx.x;
+  const Job& J = jobs[jb];
+  const int total = min(cnt->n_leaves, tile_cap);
+  int b0 = scan_blocks[J.cube_off / kScanChunk];
+  int b1 = jb + 1 < njobs ? scan_blocks[jobs[jb + 1].cube_off / kScanChunk]
+                          : cnt->n_leaves;
+  b0 = min(b0, total);
+  b1 = min(b1, total);
+  if (threadIdx.x < 7) s[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int it = threadIdx.x; it < J.nitems; it += 256)
+    for (int q = 0; q < 4; ++q) a[q] += part[(size_t)(J.item_off + it) * 4 + q];
+  for (int sl = b0 + threadIdx.x; sl < b1; sl += 256)
+    for (int q = 0; q < 3; ++q) a[4 + q] += slot_stats[(size_t)sl * 4 + q];
+  for (int q = 0; q < 7; ++q) {
+    const uint64_t w = wave_sum(a[q]);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(&s[q], (unsigned long long)w);
   }
   __syncthreads();
-  if (li < 3) {
-    uint64_t t = 0;
-    for (int w = 0; w < 8; ++w) t += s_part[w][li];
-    part[(size_t)blockIdx.x * 4 + li] = t;
+  if (threadIdx.x == 0) {
+    JobStats r;
+    r.visits = s[0];
+    r.rays = s[1];
+    r.hit_rays = s[2];
+    r.errors = s[3];
+    r.voxels = s[4];
+    r.hit_voxels = s[5];
+    r.mixed_voxels = s[6];
+    r.slot_begin = b0;
+    r.slot_end = b1;
+    out[jb] = r;
   }
 }
 
@@ -795,92 +963,143 @@ __global__ void k_map_clear(MapView m) {
   }
 }
 
+ItemQueue cast_queue(const Pipeline& p, const Workspace& b, int which,
+                     int* grid) {
+  ItemQueue q;
+  q.item_job = p.njobs > 1 ? b.item_job : nullptr;
+  q.order = nullptr;
+  q.head = nullptr;
+  q.nitems = p.nitems;
+  *grid = p.nitems;
+  if (p.tuning.queue != 0) {
+    q.head = &b.counters->queue[which];
+    if (p.tuning.queue == 2) q.order = b.item_order;
+    const int g = p.cu_count * (p.tuning.grid_mult > 0 ? p.tuning.grid_mult : 1);
+    *grid = p.nitems < g ? p.nitems : g;
+  }
+  return q;
+}
+
 }  // namespace
 
-hipError_t launch_reset_counters(FrameCounters* c, hipStream_t s) {
-  k_reset_counters<<<1, 64, 0, s>>>(c);
+int job_items(const Job& j, int layout) {
+  if (j.n <= 0) return 0;
+  if (j.tiled) {
+    int tw, th, ww;
+    layout_dims(layout, tw, th, ww);
+    return ((j.width + tw - 1) / tw) * ((j.height + th - 1) / th);
+  }
+  return (j.n + kCastThreads - 1) / kCastThreads;
+}
+
+hipError_t launch_begin(const Workspace& b, const Job* job0, hipStream_t s) {
+  Job j{};
+  if (job0) j = *job0;
+  k_begin<<<1, 64, 0, s>>>(j, job0 ? 1 : 0, b.jobs, b.counters);
   return hipGetLastError();
 }
 
-hipError_t launch_discover(int rounding, const CastParams& p,
-                           const CtxBuffers& b, hipStream_t s) {
-  if (p.n <= 0) return hipSuccess;
-  const int grid = cast_grid(p);
-  if (rounding == kDown)
-    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(p, b.leaf_slot, b.ray_hit,
-                                                    b.part_discover);
-  else
-    k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(p, b.leaf_slot,
-                                                       b.ray_hit,
-                                                       b.part_discover);
-  return hipGetLastError();
-}
-
-hipError_t launch_compact_leaves(const CastParams& p, const CtxBuffers& b,
-                                 int64_t cube_vol, int tile_cap,
-                                 hipStream_t s) {
-  const int nblocks = (int)((cube_vol + kScanChunk - 1) / kScanChunk);
-  k_scan_count<<<nblocks, 256, 0, s>>>(b.leaf_slot, cube_vol, b.scan_blocks);
-  k_scan_blocks<<<1, 1024, 0, s>>>(b.scan_blocks, nblocks, b.part_discover,
-                                   p.n > 0 ? cast_grid(p) : 0, b.counters);
-  k_scan_assign<<<nblocks, 256, 0, s>>>(b.leaf_slot, cube_vol, b.scan_blocks,
-                                        b.slot_leaf, tile_cap, b.counters);
-  return hipGetLastError();
-}
-
-hipError_t launch_hit_keys(const CastParams& p, const CtxBuffers& b,
+hipError_t launch_discover(const Pipeline& p, const Workspace& b,
                            hipStream_t s) {
-  if (p.n <= 0) return hipSuccess;
-  k_hit_keys<<<(p.n + 255) / 256, 256, 0, s>>>(p.n, b.ray_hit, b.leaf_slot,
-                                               b.keys[0], b.vals[0]);
+  if (p.nitems <= 0) return hipSuccess;
+  int grid = 0;
+  const ItemQueue q = cast_queue(p, b, 0, &grid);
+  if (p.rounding == kDown)
+    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(b.jobs, q, b.leaf_slot,
+                                                    b.ray_hit, b.part_discover);
+  else
+    k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover);
   return hipGetLastError();
 }
 
-hipError_t launch_hit_segments(const CtxBuffers& b, int n, int sorted_buf,
-                               hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int grid = min((n + 255) / 256, 2048);
+hipError_t launch_compact_leaves(const Pipeline& p, const Workspace& b,
+                                 hipStream_t s) {
+  const int nblocks = (int)((p.cube_vol + kScanChunk - 1) / kScanChunk);
+  if (nblocks <= 0) return hipSuccess;
+  k_scan_count<<<nblocks, 256, 0, s>>>(b.leaf_slot, p.cube_vol, b.scan_blocks);
+  k_scan_blocks<<<1, 1024, 0, s>>>(b.scan_blocks, nblocks, b.part_discover,
+                                   p.nitems, b.counters);
+  k_scan_assign<<<nblocks, 256, 0, s>>>(b.leaf_slot, p.cube_vol, b.scan_blocks,
+                                        b.scan_job, b.slot_leaf, b.slot_job,
+                                        p.tile_cap, b.counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_hit_keys(const Pipeline& p, const Workspace& b,
+                           hipStream_t s) {
+  if (p.nitems <= 0) return hipSuccess;
+  ItemQueue q;
+  q.item_job = p.njobs > 1 ? b.item_job : nullptr;
+  q.order = nullptr;
+  q.head = nullptr;
+  q.nitems = p.nitems;
+  k_hit_keys<<<p.nitems, 256, 0, s>>>(b.jobs, q, b.ray_hit, b.leaf_slot,
+                                      b.keys[0], b.vals[0]);
+  return hipGetLastError();
+}
+
+hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
+                               int sorted_buf, hipStream_t s) {
+  if (p.n_rays <= 0) return hipSuccess;
+  const int64_t g = (p.n_rays + 255) / 256;
+  const int grid = g < 2048 ? (int)g : 2048;
   k_hit_segments<<<grid, 256, 0, s>>>(b.keys[sorted_buf], b.counters, b.tiles,
                                       b.hit_mask, b.hit_start, b.gap,
                                       b.slot_leaf, b.leaf_slot);
   return hipGetLastError();
 }
 
-hipError_t launch_accumulate(int rounding, const CastParams& p,
-                             const CtxBuffers& b, int sorted_buf,
-                             hipStream_t s) {
-  if (p.n <= 0) return hipSuccess;
-  const int grid = cast_grid(p);
-  if (rounding == kDown)
+hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
+                             int sorted_buf, hipStream_t s) {
+  if (p.nitems <= 0) return hipSuccess;
+  int grid = 0;
+  const ItemQueue q = cast_queue(p, b, 1, &grid);
+  if (p.rounding == kDown)
     k_accumulate<kDown><<<grid, kCastThreads, 0, s>>>(
-        p, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
+        b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
         b.vals[sorted_buf], b.gap, b.counters);
   else
     k_accumulate<kNearest><<<grid, kCastThreads, 0, s>>>(
-        p, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
+        b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
         b.vals[sorted_buf], b.gap, b.counters);
   return hipGetLastError();
 }
 
-hipError_t launch_update(int rounding, const Cube& cube, const CtxBuffers& b,
-                         const MapView& m, float hit, float miss,
-                         int sorted_buf, int tile_cap, hipStream_t s) {
-  (void)sorted_buf;
-  k_map_lookup<<<256, 256, 0, s>>>(cube, b.slot_leaf, m, tile_cap, b.counters,
-                                   b.map_idx, b.map_new);
+hipError_t launch_update(const Pipeline& p, const Workspace& b,
+                         const MapView* map_table, const MapView& single,
+                         hipStream_t s) {
+  k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
+                                   single, p.tile_cap, b.counters, b.map_idx,
+                                   b.map_new);
   const int grid = kUpdateGrid;
-  if (rounding == kDown)
-    k_update<kDown><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
-                                         b.tiles, b.hit_mask, b.hit_start,
-                                         b.gap, b.map_idx, b.map_new, m, hit,
-                                         miss, tile_cap, b.counters,
-                                         b.part_update);
+  if (p.rounding == kDown)
+    k_update<kDown><<<grid, 512, 0, s>>>(
+        b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
+        b.hit_start, b.gap, b.map_idx, b.map_new, map_table, single,
+        p.tile_cap, b.counters, b.slot_stats);
   else
-    k_update<kNearest><<<grid, 512, 0, s>>>(cube, b.leaf_slot, b.slot_leaf,
-                                            b.tiles, b.hit_mask, b.hit_start,
-                                            b.gap, b.map_idx, b.map_new, m,
-                                            hit, miss, tile_cap, b.counters,
-                                            b.part_update);
+    k_update<kNearest><<<grid, 512, 0, s>>>(
+        b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
+        b.hit_start, b.gap, b.map_idx, b.map_new, map_table, single,
+        p.tile_cap, b.counters, b.slot_stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_slots(const Pipeline& p, const Workspace& b,
+                              hipStream_t s) {
+  k_clear_slots<<<kUpdateGrid, 512, 0, s>>>(b.leaf_slot, b.slot_leaf, b.tiles,
+                                            b.hit_mask, b.counters,
+                                            p.tile_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_job_stats(const Pipeline& p, const Workspace& b,
+                            hipStream_t s) {
+  if (p.njobs <= 0) return hipSuccess;
+  k_job_stats<<<p.njobs, 256, 0, s>>>(b.jobs, p.njobs, b.part_discover,
+                                      b.scan_blocks, b.slot_stats, b.counters,
+                                      p.tile_cap, b.job_stats);
   return hipGetLastError();
 }
 

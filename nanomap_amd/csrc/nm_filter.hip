@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_flag_emit(
 }  // namespace
 
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               const CtxBuffers& b, hipStream_t s) {
+                               const Workspace& b, hipStream_t s) {
   if (p.n <= 0) return hipSuccess;
   FilterKey fk;
   const int need = (int)__builtin_ceilf(p.max_time) + 3;

@@ -16,20 +16,27 @@ constexpr uint32_t kInvalid = 0xffffffffu;
 constexpr int kLeafVoxels = 512;  // 8^3
 constexpr int kMaskWords = 16;    // 512 bits
 
-// Device-side counters of one frame.  Zeroed by a fill kernel at the start
-// of each frame (no hipMemsetAsync, so the sequence can be graph-captured).
+// Device-side counters of one launch sequence (one frame, or one batch of
+// frames).  Zeroed by the first kernel of the sequence (no hipMemsetAsync,
+// so the sequence can be graph-captured).
 struct FrameCounters {
-  int32_t n_leaves;    // leaves discovered (tile slots in use)
-  int32_t n_hit_rays;  // rays whose end voxel received a hit
+  int32_t n_leaves;    // leaves discovered (tile slots in use), all jobs
+  int32_t n_hit_rays;  // rays whose end voxel received a hit, all jobs
   int32_t errors;      // consistency errors (expected 0)
   int32_t overflow;    // tile or map capacity exceeded
   uint64_t rays;       // valid rays
   uint64_t visits;     // voxel updates
-  uint64_t voxels;     // unique voxels touched
-  uint64_t hit_voxels;
-  uint64_t mixed_voxels;
   int32_t n_filter_rays;    // rays emitted by the voxel filter
   int32_t n_filter_points;  // points binned by the voxel filter
+  int32_t queue[4];         // work-queue heads of the persistent kernels
+};
+
+// Statistics of one job, reduced on demand from per-item and per-slot
+// partial sums (k_job_stats).
+struct JobStats {
+  unsigned long long rays, visits, hit_rays, errors;
+  unsigned long long voxels, hit_voxels, mixed_voxels;
+  int32_t slot_begin, slot_end;  // tile slots of this job
 };
 
 // The sparse map on the device: an open-addressing hash of packed leaf
@@ -58,8 +65,20 @@ struct Cube {
   int side;
 };
 
-// Per-frame parameters of the cast kernels (passed by value).
-struct CastParams {
+// Pixel-tile layouts of the cast work items (256 rays each, four
+// wavefronts): width x height of the item and of each wavefront's part.
+enum Layout : int {
+  kLayout16x16 = 0,  // 2x2 wavefronts of 8x8
+  kLayout32x8 = 1,   // 4x1 wavefronts of 8x8
+  kLayout64x4 = 2,   // 4x1 wavefronts of 16x4
+  kLayout8x32 = 3,   // 1x4 wavefronts of 8x8
+  kLayoutCount = 4
+};
+
+// One frame of one sensor ("job").  The cast kernels read jobs from a
+// device table, so one launch serves any number of sensors; a plain frame
+// is a table of one job.
+struct Job {
   float R[9];
   double eye[3];
   float res;
@@ -76,32 +95,72 @@ struct CastParams {
   int height;
   int tiled;                   // 1: points form a width x height image
   int use_rays;                // 1: cast rays4 instead of points
+  int layout;                  // Layout of the work items (tiled only)
+  int map;                     // index into the map table
+  // placement in the shared workspace
+  int64_t cube_off;  // first entry of the job's cube table (kScanChunk-aligned)
+  int ray_off;       // global index of the job's first point
+  int item_off;      // first cast work item of the job
+  int nitems;        // cast work items of the job
+  int pad_;
+};
+using CastParams = Job;
+
+// How the cast kernels walk their work items (chosen by the autotuner for
+// batches; plain frames use the default).
+struct Tuning {
+  int layout = kLayout16x16;
+  int queue = 0;      // 0: one workgroup per item; 1: persistent workgroups
+                      // draining a queue in job order; 2: same, costliest
+                      // items first
+  int grid_mult = 4;  // persistent workgroups per compute unit
 };
 
-// Device workspace of one context (one sensor stream).
-struct CtxBuffers {
+// Device workspace shared by all jobs of a launch sequence.
+struct Workspace {
   int32_t* leaf_slot;   // [cube_cap]  -1 free, -2 discovered, >= 0 slot
                         //             (| kSlotHasHits once hits are sorted)
-  int32_t* slot_leaf;   // [tile_cap]  cube index of each slot
+  int32_t* slot_leaf;   // [tile_cap]  workspace cube index of each slot
+  int32_t* slot_job;    // [tile_cap]  job of each slot
   int32_t* map_idx;     // [tile_cap]  map leaf of each slot, -1 if new
   int32_t* map_new;     // [tile_cap]  pool index reserved for a new leaf
+  uint32_t* slot_stats; // [tile_cap * 4]  voxels, hit voxels, mixed voxels
   uint32_t* tiles;      // [tile_cap * 512 * 2]  (miss, hit) per voxel
   uint32_t* hit_mask;   // [tile_cap * 16]  voxels holding a hit
   uint32_t* hit_start;  // [tile_cap * 512] first sorted hit of a hit voxel
-  uint32_t* ray_hit;    // [max_points]  cube cell of each ray's hit
-  uint32_t* keys[2];    // [max_points]  radix sort buffers
+  uint32_t* ray_hit;    // [ray_cap]  job-cube cell of each ray's hit
+  uint32_t* keys[2];    // [ray_cap]  radix sort buffers
   uint32_t* vals[2];
-  uint32_t* gap;        // [max_points]  misses before each sorted hit
-  uint32_t* radix_hist; // [256 * max_radix_blocks]
-  int32_t* scan_blocks; // [cube_cap / kScanChunk + 1]
+  uint32_t* gap;        // [ray_cap]  misses before each sorted hit
+  uint32_t* radix_hist; // radix_hist_words(ray_cap)
+  int32_t* scan_blocks; // [cube_cap / kScanChunk + 2]
+  int32_t* scan_job;    // [cube_cap / kScanChunk + 2]  job of each chunk
   FrameCounters* counters;
-  uint64_t* part_discover;  // [discover blocks * 4] per-block counter sums
-  uint64_t* part_update;    // [kUpdateGrid * 4] voxels, hit, mixed
-  unsigned char* cloud; // [max_points * 16] staging for host clouds
-  float* filter_rays;   // [max_points * 4]
-  float* filter_tmp;    // [max_points * 4]
-  int32_t* flags;       // [max_points]
+  JobStats* job_stats;  // [max_jobs]
+  Job* jobs;            // [max_jobs]  device job table
+  MapView* maps;        // [max_jobs]  device map table
+  uint64_t* part_discover;  // [item_cap * 4] per-item counter sums
+  int32_t* item_job;    // [item_cap]  job of each work item
+  int32_t* item_order;  // [item_cap]  queue order of the work items
+  unsigned char* cloud; // staging for host clouds
+  float* filter_rays;   // [ray_cap * 4]
+  float* filter_tmp;    // [ray_cap * 4]
+  int32_t* flags;       // [ray_cap]
 };
+
+// Sizes of a workspace.
+struct WorkspaceDims {
+  int64_t cube_cap = 0;  // entries of all jobs' cube tables
+  int tile_cap = 0;      // tile slots
+  int64_t ray_cap = 0;   // points of all jobs
+  int item_cap = 0;      // cast work items of all jobs
+  int max_jobs = 1;
+  bool host_clouds = false;  // staging for host clouds + voxel filter
+};
+
+// Allocates a workspace (cube tables free, tiles zero); synchronous.
+hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out);
+void workspace_free(Workspace* b);
 
 constexpr int kScanChunk = 4096;
 // Bit of a cube-table entry marking a leaf that holds hit voxels (set after
@@ -113,23 +172,49 @@ constexpr int kRadixTile = 2048;
 constexpr uint32_t kHeavyHits = 256;
 
 // ---- launchers (nm_cast.hip, nm_radix.hip, nm_filter.hip) -------------
-hipError_t launch_reset_counters(FrameCounters* c, hipStream_t s);
-hipError_t launch_discover(int rounding, const CastParams& p,
-                           const CtxBuffers& b, hipStream_t s);
-hipError_t launch_compact_leaves(const CastParams& p, const CtxBuffers& b,
-                                 int64_t cube_vol, int tile_cap,
-                                 hipStream_t s);
-hipError_t launch_hit_keys(const CastParams& p, const CtxBuffers& b,
+// The frame sequence over a device job table `b.jobs[0, njobs)`:
+//   begin -> discover -> compact -> hit keys -> radix sort -> segments ->
+//   accumulate -> map lookup -> update (or clear, without a map).
+// `host_jobs` is the host copy of the table (grid sizes are taken from
+// it); `job0` non-null makes the begin kernel store that job as entry 0
+// (plain frames, no separate upload).
+struct Pipeline {
+  const Job* host_jobs;
+  int njobs;
+  int64_t cube_vol;   // workspace cube entries in use
+  int64_t n_rays;     // points of all jobs
+  int nitems;         // cast work items of all jobs
+  int tile_cap;
+  int sort_bits;
+  int rounding;
+  int cu_count;
+  Tuning tuning;
+};
+
+hipError_t launch_begin(const Workspace& b, const Job* job0, hipStream_t s);
+hipError_t launch_discover(const Pipeline& p, const Workspace& b,
                            hipStream_t s);
-hipError_t launch_hit_segments(const CtxBuffers& b, int n, int sorted_buf,
-                               hipStream_t s);
-hipError_t launch_accumulate(int rounding, const CastParams& p,
-                             const CtxBuffers& b, int sorted_buf,
-                             hipStream_t s);
-hipError_t launch_update(int rounding, const Cube& cube, const CtxBuffers& b,
-                         const MapView& m, float hit, float miss,
-                         int sorted_buf, int tile_cap, hipStream_t s);
+hipError_t launch_compact_leaves(const Pipeline& p, const Workspace& b,
+                                 hipStream_t s);
+hipError_t launch_hit_keys(const Pipeline& p, const Workspace& b,
+                           hipStream_t s);
+hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
+                               int sorted_buf, hipStream_t s);
+hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
+                             int sorted_buf, hipStream_t s);
+// map_table: device table indexed by Job::map, or null to use `single`.
+hipError_t launch_update(const Pipeline& p, const Workspace& b,
+                         const MapView* map_table, const MapView& single,
+                         hipStream_t s);
+// Clears the per-frame state of the last cast without touching a map.
+hipError_t launch_clear_slots(const Pipeline& p, const Workspace& b,
+                              hipStream_t s);
+hipError_t launch_job_stats(const Pipeline& p, const Workspace& b,
+                            hipStream_t s);
 hipError_t launch_map_clear(const MapView& m, hipStream_t s);
+
+// Cast work items of a job under a layout.
+int job_items(const Job& j, int layout);
 
 // Stable LSD radix sort of (keys, vals) over the low `bits` key bits.
 // Sorts buffer 0 -> result in buffer returned via *out_buf (0 or 1).
@@ -144,7 +229,7 @@ int radix_hist_words(int n);  // histogram buffer size for n keys
 // part of the eye, eye - floor(eye), as FP32 (the reference's
 // voxelOriginOffset, include/nanomap/sensor/SensorData.h:55).
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               const CtxBuffers& b, hipStream_t s);
+                               const Workspace& b, hipStream_t s);
 
 void set_error(const std::string& msg);
 
