@@ -11,10 +11,11 @@ import ctypes
 import numpy as np
 
 from ._capi import (FILTER_NONE, FILTER_VOXEL, ROUND_DOWN, ROUND_NEAREST,
-                    Camera, CtxConfig, Frame, FrameStats, NanoMapError,
-                    SimSensor, _ptr, check, lib)
+                    AgentFrame, BatchConfig, BatchTuning, Camera, CtxConfig,
+                    Frame, FrameStats, NanoMapError, SimSensor, _ptr, check,
+                    lib)
 
-__all__ = ["Context", "Map", "Scene", "Camera", "Frame", "FrameStats",
+__all__ = ["Context", "Map", "Batch", "Scene", "Camera", "Frame", "FrameStats",
            "SimGrid", "SimSensor", "sim_sensor",
            "frame_from_pose", "fold", "NanoMapError", "ROUND_DOWN",
            "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "DEFAULT_F"]
@@ -215,6 +216,83 @@ class Map:
         o, v, m = o[:L], v[:L], m[:L]
         order = np.lexsort((o[:, 2], o[:, 1], o[:, 0]))
         return o[order], v[order], m[order]
+
+
+class Batch:
+    """Many agents' frames in one graph-captured launch sequence
+    (include/nanomap_amd_batch.h).  Frames are (frame, cloud, map) with the
+    cloud device-resident: a torch CUDA tensor [n, 3|4] float32 or a
+    (device_ptr, n_points, step) tuple."""
+
+    def __init__(self, max_frames, max_points, max_time, tile_capacity,
+                 device=0):
+        cfg = BatchConfig(device, int(max_frames), int(max_points),
+                          float(max_time), int(tile_capacity))
+        h = ctypes.c_void_p()
+        check(lib().nm_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.n = 0
+
+    def close(self):
+        if self.handle:
+            lib().nm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _table(frames):
+        arr = (AgentFrame * len(frames))()
+        keep = []
+        for i, (fr, cloud, m) in enumerate(frames):
+            ptr, n, st, dev = Context._cloud_args(cloud, 0)
+            if not dev:
+                raise NanoMapError(1, "Batch: clouds must be on the device")
+            arr[i].frame = fr
+            arr[i].cloud = ptr
+            arr[i].n_points = n
+            arr[i].step = st
+            arr[i].map = m.handle
+            keep.append(cloud)
+        return arr, keep
+
+    def submit(self, frames, stream=None):
+        arr, keep = self._table(frames)
+        self._keep = keep
+        check(lib().nm_batch_submit(self.handle, arr, len(frames), stream))
+        self.n = len(frames)
+
+    def stats(self):
+        out = (FrameStats * self.n)()
+        check(lib().nm_batch_stats(self.handle, out, self.n))
+        return [o.as_dict() for o in out]
+
+    def tune(self, frames, reps=2, stream=None):
+        """Autotune the cast kernels on these frames; returns (tuning dict,
+        ms per batch of the fastest candidate)."""
+        arr, keep = self._table(frames)
+        best = BatchTuning()
+        ms = ctypes.c_float()
+        check(lib().nm_batch_tune(self.handle, arr, len(frames), int(reps),
+                                  stream, ctypes.byref(best),
+                                  ctypes.byref(ms)))
+        return best.as_dict(), ms.value
+
+    def set_tuning(self, layout=0, queue=0, grid_mult=4):
+        t = BatchTuning(int(layout), int(queue), int(grid_mult))
+        check(lib().nm_batch_set_tuning(self.handle, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = BatchTuning()
+        check(lib().nm_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_graph(self, enable):
+        check(lib().nm_batch_set_graph(self.handle, int(bool(enable))))
 
 
 class Scene:

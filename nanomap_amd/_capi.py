@@ -1,4 +1,5 @@
-"""ctypes binding of the C ABI in include/nanomap_amd.h and
+"""ctypes binding of the C ABI in include/nanomap_amd.h,
+include/nanomap_amd_batch.h, include/nanomap_amd_sim.h and
 include/nanomap_amd_scene.h.
 
 The shared library is built in-tree (nanomap_amd/libnanomap_amd.so).  If it
@@ -63,6 +64,26 @@ class FrameStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("max_frames", ctypes.c_int),
+                ("max_points", ctypes.c_int64), ("max_time", ctypes.c_float),
+                ("tile_capacity", ctypes.c_int64)]
+
+
+class AgentFrame(ctypes.Structure):
+    _fields_ = [("frame", Frame), ("cloud", ctypes.c_void_p),
+                ("n_points", ctypes.c_int), ("step", ctypes.c_int),
+                ("map", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_int), ("queue", ctypes.c_int),
+                ("grid_mult", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SimSensor(ctypes.Structure):
     _fields_ = [("position", ctypes.c_float * 3),
                 ("rotation", ctypes.c_float * 9), ("grid_res", ctypes.c_float),
@@ -114,6 +135,16 @@ SIGNATURES = {
     "nm_ctx_set_timing": (_I, [_P, _I]),
     "nm_ctx_stage_times": (_I, [_P, _PF, _PI64]),
     "nm_ctx_reset_timing": (_I, [_P]),
+    "nm_batch_create": (_I, [ctypes.POINTER(BatchConfig),
+                             ctypes.POINTER(_P)]),
+    "nm_batch_destroy": (_I, [_P]),
+    "nm_batch_submit": (_I, [_P, ctypes.POINTER(AgentFrame), _I, _P]),
+    "nm_batch_stats": (_I, [_P, ctypes.POINTER(FrameStats), _I]),
+    "nm_batch_tune": (_I, [_P, ctypes.POINTER(AgentFrame), _I, _I, _P,
+                           ctypes.POINTER(BatchTuning), _PF]),
+    "nm_batch_set_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
+    "nm_batch_get_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
+    "nm_batch_set_graph": (_I, [_P, _I]),
     "nm_scene_create": (_I, [ctypes.c_uint64, _PD, _I, _D, _D, _I, _D, _D,
                              ctypes.POINTER(_P)]),
     "nm_scene_destroy": (_I, [_P]),

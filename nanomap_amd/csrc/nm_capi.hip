@@ -61,13 +61,6 @@ struct nm_ctx {
   int64_t stage_n[NM_STAGE_COUNT] = {};
 };
 
-struct nm_map {
-  int device = 0;
-  float grid_res = 0;
-  nm::MapView v{};
-  hipStream_t stream = nullptr;
-};
-
 namespace nm {
 
 static thread_local std::string g_error;
@@ -96,23 +89,6 @@ int hip_fail(hipError_t e, const char* what) {
 template <class T>
 hipError_t dalloc(T** p, size_t count) {
   return hipMalloc((void**)p, count * sizeof(T) + 64);
-}
-
-int sort_bits_for(int64_t max_key) {
-  int bits = 1;
-  while (bits < 32 && ((int64_t)1 << bits) <= max_key) ++bits;
-  return bits;
-}
-
-nm::Cube frame_cube(const nm_frame& f) {
-  nm::Cube c;
-  const int r = (int)ceil(((double)f.max_time + 2.0) / 8.0) + 1;
-  c.side = 2 * r + 1;
-  for (int a = 0; a < 3; ++a) {
-    const int v = (int)floor(f.eye[a]);
-    c.cmin[a] = (v >> 3) - r;
-  }
-  return c;
 }
 
 hipEvent_t take_event(nm_ctx* c) {
@@ -403,7 +379,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     return fail(NM_ESTATE, "nm_cast: the previous cast was not folded into a map (nm_update_map)");
   NM_HIP(hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  const nm::Cube cube = frame_cube(*frame);
+  const nm::Cube cube = nm::frame_cube(*frame);
   if (cube.side > ctx->cube_side_max)
     return fail(NM_EINVAL, "nm_cast: frame cube exceeds the context");
 
@@ -457,7 +433,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   pl.n_rays = n_points;
   pl.nitems = p.nitems;
   pl.tile_cap = ctx->tile_cap;
-  pl.sort_bits = sort_bits_for((int64_t)ctx->tile_cap * 512);
+  pl.sort_bits = nm::sort_bits_for((int64_t)ctx->tile_cap * 512);
   pl.rounding = frame->rounding;
   pl.cu_count = ctx->cu_count;
   ctx->job = p;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <string>
 
 #include "../../include/nanomap_amd.h"
@@ -233,6 +234,34 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
 
 void set_error(const std::string& msg);
 
+// Bits needed for keys below max_key.
+inline int sort_bits_for(int64_t max_key) {
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) <= max_key) ++bits;
+  return bits;
+}
+
+// Cube of candidate leaves of a frame: every voxel within max_time of the
+// eye, with one leaf of margin.
+inline Cube frame_cube(const nm_frame& f) {
+  Cube c;
+  const int r = (int)ceil(((double)f.max_time + 2.0) / 8.0) + 1;
+  c.side = 2 * r + 1;
+  for (int a = 0; a < 3; ++a) {
+    const int v = (int)floor(f.eye[a]);
+    c.cmin[a] = (v >> 3) - r;
+  }
+  return c;
+}
+
 }  // namespace nm
+
+// The device-resident map behind the opaque nm_map handle.
+struct nm_map {
+  int device = 0;
+  float grid_res = 0;
+  nm::MapView v{};
+  hipStream_t stream = nullptr;
+};
 
 #endif  // NM_INTERNAL_H

@@ -5,6 +5,11 @@ cfg1: 20x20x5 m room, 40 boxes (sides 0.3-2 m), one 160x120 frame at
 cfg2: 50x50x10 m room, 300 boxes + 200 cylinders (r 0.1-0.5 m), a
       100-frame random walk (0.1 m, 2 deg per frame), 640x480 at 87x58 deg,
       10 m range, 0.05 m voxels, 5% NaN dropouts.
+cfg4: 32 agents in a 100x100x10 m room (1200 boxes + 800 cylinders), each
+      with its own pose, sensor resolution from {160x120, 320x240, 640x480,
+      848x480, 1280x720} (87x58 deg for 4:3, 90x60 deg for 16:9), range
+      uniform 3-20 m, voxel size from {0.05, 0.1, 0.2} m, all from the seed;
+      depth noise and dropouts as cfg2.
 Log-odds hit +0.85, miss -0.4, clamp [-2.0, 3.5], thresholds 0/0.
 """
 import numpy as np
@@ -63,3 +68,42 @@ def cfg1_map_leaves(nm, oracle_map=None):
     ctx.submit(m, f, cloud)
     o, v, _ = m.download()
     return o, v
+
+
+CFG4_RESOLUTIONS = ((160, 120), (320, 240), (640, 480), (848, 480),
+                    (1280, 720))
+CFG4_VOXELS = (0.05, 0.1, 0.2)
+
+
+def cfg4_scene(nm, seed=4):
+    """100x100x10 m room with 1200 boxes and 800 cylinders."""
+    return nm.Scene(seed, (100.0, 100.0, 10.0), 1200, 0.3, 2.0, 800, 0.1, 0.5)
+
+
+def cfg4_agents(nm, n_agents=32, seed=4, scene=None, frame_index=0):
+    """Per agent: dict(width, height, hfov, vfov, range, res, pos, quat,
+    cloud).  frame_index > 0 moves every agent one random-walk step per
+    index (0.1 m, 2 deg), so consecutive batches are new frames."""
+    scene = scene or cfg4_scene(nm, seed)
+    rng = np.random.RandomState(seed)
+    out = []
+    for k in range(n_agents):
+        w, h = CFG4_RESOLUTIONS[rng.randint(len(CFG4_RESOLUTIONS))]
+        rmax = float(rng.uniform(3.0, 20.0))
+        res = float(CFG4_VOXELS[rng.randint(len(CFG4_VOXELS))])
+        hf, vf = (87.0, 58.0) if w * 3 == h * 4 else (90.0, 60.0)
+        pos, quat = scene.walk(seed * 1000 + k, frame_index + 1, 0.1, 2.0,
+                               15.0, 0.3)
+        pos, quat = pos[frame_index], quat[frame_index]
+        cam = nm.camera(w, h, hf, vf, rmax, noise_k=0.002, dropout=0.05)
+        cloud = scene.render(cam, pos, quat, seed * 100000 + k * 100 +
+                             frame_index)
+        out.append(dict(width=w, height=h, hfov=hf, vfov=vf, range=rmax,
+                        res=res, pos=pos, quat=quat, cloud=cloud))
+    return scene, out
+
+
+def agent_frame(nm, a, rounding):
+    return nm.frame_from_pose(a["pos"], a["quat"], nm.DEFAULT_F, a["res"],
+                              a["range"], HIT, MISS, rounding, a["width"],
+                              a["height"])
