@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=30,
                     help="frames of the CPU baseline sample (0 = skip)")
     ap.add_argument("--rounding", choices=["down", "nearest"], default="down")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="timed passes of the 32-agent batch (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed steps, run one fill and one "
                          "elementwise copy of this many MiB (the known-size "
@@ -62,6 +64,80 @@ def measured_traffic(stage):
 def percentile(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def bench_batch(nm, W, torch, steps, rounding):
+    """32 agents (uneven image sizes, ranges, voxel sizes; own maps): one
+    graph-captured batched launch per pass, against the same frames as 32
+    plain launches one after another, on the same stream."""
+    _, agents = W.cfg4_agents(nm)
+    clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
+    frames = [W.agent_frame(nm, a, rounding) for a in agents]
+    P = max(c.shape[0] for c in clouds)
+    T = max(f.max_time for f in frames) + 1.0
+    total = sum(c.shape[0] for c in clouds)
+
+    def maps():
+        cap = {0.05: 1 << 19, 0.1: 1 << 17}
+        return [nm.Map(a["res"], *W.CLAMP,
+                       leaf_capacity=cap.get(a["res"], 1 << 16))
+                for a in agents]
+
+    stream = torch.cuda.current_stream()
+    sh = ctypes_stream(stream)
+    ctx = nm.Context(P, T)
+    pmaps = maps()
+    rays = 0
+    for f, c, m in zip(frames, clouds, pmaps):  # untimed pass: ray counts
+        ctx.submit(m, f, c, stream=sh)
+        rays += ctx.stats()["rays"]
+    batch = nm.Batch(len(agents), total, T, 1 << 21)
+    bmaps = maps()
+    table = list(zip(frames, clouds, bmaps))
+    t0 = time.perf_counter()
+    tuning, tune_ms = batch.tune(table, reps=2, stream=sh)
+    tune_s = time.perf_counter() - t0
+    batch.submit(table, stream=sh)  # layout + graph capture
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    def plain_pass():
+        for f, c, m in zip(frames, clouds, pmaps):
+            ctx.submit(m, f, c, stream=sh)
+
+    plain_ms = timed(plain_pass)
+    batch_ms = timed(lambda: batch.submit(table, stream=sh))
+    st = batch.stats()
+    assert all(s["errors"] == 0 and s["overflow"] == 0 for s in st)
+    visits = sum(s["visits"] for s in st)
+    return {
+        "workload": "32 agents in a 100x100x10 m room (1200 boxes + 800 "
+                    "cylinders): image sizes from {160x120, 320x240, "
+                    "640x480, 848x480, 1280x720}, ranges 3-20 m, voxels "
+                    "{0.05, 0.1, 0.2} m, own maps; clouds resident in HBM",
+        "agents": len(agents),
+        "points": int(total),
+        "rays": int(rays),
+        "plain_ms": plain_ms,
+        "batched_ms": batch_ms,
+        "visits": int(visits),
+        "batched_gvisits_per_s": visits / batch_ms / 1e6,
+        "plain_mrays_per_s": rays / plain_ms / 1e3,
+        "batched_mrays_per_s": rays / batch_ms / 1e3,
+        "speedup": plain_ms / batch_ms,
+        "tuning": tuning,
+        "tuner_s": tune_s,
+        "steps": steps,
+    }
 
 
 def cpu_baseline(frames_params, clouds, n):
@@ -102,8 +178,9 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not "
-              "supported (agents and sensors shard across GPUs without "
-              "exchange; see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "supported: every run here uses one GPU (agents and sensors "
+              "would shard across GPUs without exchange)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -260,6 +337,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    del dev, ctx, gmap
+    if args.batch_steps > 0:
+        out["batched"] = bench_batch(nm, W, torch, args.batch_steps, rounding)
+        print("batched done", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(params, clouds,
                                            min(args.cpu_frames, args.frames))

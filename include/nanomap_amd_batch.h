@@ -48,6 +48,8 @@ typedef struct nm_batch_tuning {
                  /* draining a device queue in frame order; 2: the same,     */
                  /* longest-range frames first                               */
   int grid_mult; /* persistent workgroups per compute unit (queue 1, 2)      */
+  int lds_slots; /* LDS aggregation table of the accumulate pass: 4096 or   */
+                 /* 2048 entries (fewer entries, more resident workgroups)  */
 } nm_batch_tuning;
 
 int nm_batch_create(const nm_batch_config* cfg, nm_batch** out);

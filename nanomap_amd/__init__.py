@@ -282,8 +282,9 @@ class Batch:
                                   ctypes.byref(ms)))
         return best.as_dict(), ms.value
 
-    def set_tuning(self, layout=0, queue=0, grid_mult=4):
-        t = BatchTuning(int(layout), int(queue), int(grid_mult))
+    def set_tuning(self, layout=0, queue=0, grid_mult=4, lds_slots=4096):
+        t = BatchTuning(int(layout), int(queue), int(grid_mult),
+                        int(lds_slots))
         check(lib().nm_batch_set_tuning(self.handle, ctypes.byref(t)))
 
     def get_tuning(self):

@@ -78,7 +78,7 @@ class AgentFrame(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("layout", ctypes.c_int), ("queue", ctypes.c_int),
-                ("grid_mult", ctypes.c_int)]
+                ("grid_mult", ctypes.c_int), ("lds_slots", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -272,7 +272,7 @@ bool layout_matches(nm_batch* b, const nm_agent_frame* f, int n) {
     return false;
   const nm::Tuning& t = b->layout_tuning;
   if (t.layout != b->tuning.layout || t.queue != b->tuning.queue ||
-      t.grid_mult != b->tuning.grid_mult)
+      t.grid_mult != b->tuning.grid_mult || t.lds_slots != b->tuning.lds_slots)
     return false;
   return shape_of(f, n) == b->shape;
 }
@@ -499,18 +499,20 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
   BH(hipSetDevice(b->device));
   hipStream_t q = (hipStream_t)stream;
   std::vector<nm::Tuning> cand;
-  for (int layout = 0; layout < nm::kLayoutCount; ++layout) {
-    nm::Tuning t;
-    t.layout = layout;
-    t.queue = 0;
-    cand.push_back(t);
-    for (int queue = 1; queue <= 2; ++queue)
-      for (int gm : {2, 4, 8}) {
-        t.queue = queue;
-        t.grid_mult = gm;
-        cand.push_back(t);
-      }
-  }
+  for (int lds : {4096, 2048})
+    for (int layout = 0; layout < nm::kLayoutCount; ++layout) {
+      nm::Tuning t;
+      t.layout = layout;
+      t.lds_slots = lds;
+      t.queue = 0;
+      cand.push_back(t);
+      for (int queue = 1; queue <= 2; ++queue)
+        for (int gm : {2, 4, 8}) {
+          t.queue = queue;
+          t.grid_mult = gm;
+          cand.push_back(t);
+        }
+    }
   const nm::Tuning saved = b->tuning;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BH(hipEventCreate(&e0));
@@ -551,6 +553,7 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
     best->layout = best_c.layout;
     best->queue = best_c.queue;
     best->grid_mult = best_c.grid_mult;
+    best->lds_slots = best_c.lds_slots;
   }
   if (best_ms) *best_ms = best_t;
   return NM_OK;
@@ -559,11 +562,13 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
 int nm_batch_set_tuning(nm_batch* b, const nm_batch_tuning* t) {
   if (!b || !t) return bfail(NM_EINVAL, "nm_batch_set_tuning: null argument");
   if (t->layout < 0 || t->layout >= nm::kLayoutCount || t->queue < 0 ||
-      t->queue > 2 || t->grid_mult < 1 || t->grid_mult > 64)
+      t->queue > 2 || t->grid_mult < 1 || t->grid_mult > 64 ||
+      (t->lds_slots != 4096 && t->lds_slots != 2048))
     return bfail(NM_EINVAL, "nm_batch_set_tuning: bad tuning");
   b->tuning.layout = t->layout;
   b->tuning.queue = t->queue;
   b->tuning.grid_mult = t->grid_mult;
+  b->tuning.lds_slots = t->lds_slots;
   return NM_OK;
 }
 
@@ -572,6 +577,7 @@ int nm_batch_get_tuning(nm_batch* b, nm_batch_tuning* t) {
   t->layout = b->tuning.layout;
   t->queue = b->tuning.queue;
   t->grid_mult = b->tuning.grid_mult;
+  t->lds_slots = b->tuning.lds_slots;
   return NM_OK;
 }
 

@@ -24,10 +24,11 @@
 //               that also holds hits increments the gap counter of the
 //               first later hit (binary search in the sorted hits), so the
 //               order of hits and misses in pixel order is known exactly.
-//  6 update     one workgroup per tile: the temp value of each voxel is
+//  6 temp       one lane per tile voxel: the temp value of each voxel is
 //               the FP32 sum of its events in pixel order, rebuilt from
-//               the counts with nm::fold (exact, any rounding mode); then
-//               map += temp, clamp, activity thresholds; tiles cleared.
+//               the counts with nm::fold (exact, any rounding mode).
+//  7 apply      one workgroup per tile: map += temp, clamp, activity
+//               thresholds; tiles cleared.
 //
 // Counts are integers, so the result depends neither on the order in which
 // lanes run nor on how the work items are shaped and scheduled: each job's
@@ -401,36 +402,45 @@ __global__ __launch_bounds__(256) void k_hit_segments(
 // per work item, at flush.  A visit that finds no free slot within kProbe
 // probes falls back to a global atomic.  Two unrolled probes measured
 // fastest: longer probe loops diverge across the wavefront and cost more
-// than the rare global fallback; 2048 or 8192 slots lose to 4096 (more
-// collisions, or less occupancy).
-constexpr int kLdsSlots = 4096;
+// than the rare global fallback.  For one frame 4096 slots beat 2048 (fewer
+// collisions) and 8192 (occupancy); a batch, with many more work items in
+// flight, can prefer 2048 slots for the extra resident workgroups -- the
+// table size is a template parameter (S = log2 slots).
 constexpr int kProbe = 2;
 
+template <int S>
 __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
                                           uint32_t* tiles, uint32_t cell,
                                           uint32_t inc) {
-  uint32_t h = (cell * 2654435761u) >> 20;  // 12-bit multiplicative hash
+  uint32_t h = (cell * 2654435761u) >> (32 - S);  // multiplicative hash
+  int found = -1;
 #pragma unroll
   for (int q = 0; q < kProbe; ++q) {
-    const uint32_t k = s_key[h];
-    if (k == cell) {
-      atomicAdd(&s_cnt[h], inc);
-      return;
-    }
-    if (k == kInvalid) {
-      const uint32_t old = atomicCAS(&s_key[h], kInvalid, cell);
-      if (old == kInvalid || old == cell) {
-        atomicAdd(&s_cnt[h], inc);
-        return;
+    if (found < 0) {
+      const uint32_t k = s_key[h];
+      if (k == cell) {
+        found = (int)h;
+      } else if (k == kInvalid) {
+        const uint32_t old = atomicCAS(&s_key[h], kInvalid, cell);
+        if (old == kInvalid || old == cell) found = (int)h;
       }
+      h = (h + 1) & ((1u << S) - 1);
     }
-    h = (h + 1) & (kLdsSlots - 1);
   }
-  atomicAdd(&tiles[2 * (size_t)cell], inc);
+  // The two adds differ in scope on purpose: the LDS one is a workgroup
+  // atomic (ds_add), the fallback a device one; with equal scopes the
+  // compiler merges them into one flat atomic, which also waits on the
+  // global memory counter.
+  if (found >= 0)
+    __hip_atomic_fetch_add(&s_cnt[found], inc, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    atomicAdd(&tiles[2 * (size_t)cell], inc);
 }
 
 // lds_count for the active lanes; when they all name the same voxel (the
 // voxels next to the eye) one lane adds the lane count.
+template <int S>
 __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
                                                uint32_t* s_cnt,
                                                uint32_t* tiles,
@@ -441,9 +451,9 @@ __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
   if (same == act) {
     const int lane = threadIdx.x & 63;
     if (lane == (int)__builtin_ctzll(act))
-      lds_count(s_key, s_cnt, tiles, cell, (uint32_t)__popcll(act));
+      lds_count<S>(s_key, s_cnt, tiles, cell, (uint32_t)__popcll(act));
   } else {
-    lds_count(s_key, s_cnt, tiles, cell, 1u);
+    lds_count<S>(s_key, s_cnt, tiles, cell, 1u);
   }
 }
 
@@ -455,7 +465,7 @@ __device__ __forceinline__ int slot_of(const Cube& cube,
   return c >= 0 ? leaf_slot[c] : -1;
 }
 
-template <int M>
+template <int M, int S>
 __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const Job* __restrict__ jobs, ItemQueue q,
     const int32_t* __restrict__ leaf_slot, uint32_t* __restrict__ tiles,
@@ -463,9 +473,10 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
     FrameCounters* __restrict__ cnt) {
-  __shared__ uint32_t s_key[kLdsSlots];
-  __shared__ uint32_t s_cnt[kLdsSlots];
-  for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
+  constexpr int kSlots = 1 << S;
+  __shared__ uint32_t s_key[kSlots];
+  __shared__ uint32_t s_cnt[kSlots];
+  for (int k = threadIdx.x; k < kSlots; k += kCastThreads) {
     s_key[k] = kInvalid;
     s_cnt[k] = 0;
   }
@@ -515,7 +526,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
             if (lo < s0 + j) atomicAdd(&gap[lo], 1u);
             atomicAdd(&tiles[2 * (size_t)cell], 1u);
           } else {
-            wave_lds_count(s_key, s_cnt, tiles, cell);
+            wave_lds_count<S>(s_key, s_cnt, tiles, cell);
           }
         }
         if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
@@ -527,7 +538,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     }
     __syncthreads();
     // flush the item's counts and leave the table empty for the next one
-    for (int k = threadIdx.x; k < kLdsSlots; k += kCastThreads) {
+    for (int k = threadIdx.x; k < kSlots; k += kCastThreads) {
       const uint32_t key = s_key[k];
       if (key != kInvalid) {
         atomicAdd(&tiles[2 * (size_t)key], s_cnt[k]);
@@ -719,102 +730,141 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
   return acc;
 }
 
+// Bits of a tile's hit word after k_temp: the voxel was touched, and it
+// had misses (its miss word then holds the FP32 temp value).
+constexpr uint32_t kTouchedBit = 1u << 30;
+constexpr uint32_t kMixedBit = 1u << 31;
+constexpr uint32_t kHitsMask = kTouchedBit - 1;
+
+// Temp value of every touched voxel: the FP32 pixel-order sum of its
+// events, rebuilt from the counts (Handler.cpp:71-82).  One lane per voxel
+// over all tiles, so the serial sums of many voxels overlap.  Voxels with
+// many hits (a surface close to the sensor, often a whole plane of a leaf)
+// are listed for k_heavy, which gives each its own wavefront.  The result
+// replaces the voxel's miss count in the tile.
 template <int M>
-__global__ __launch_bounds__(512, 4) void k_update(
+__global__ __launch_bounds__(256) void k_temp(
+    const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
+    uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
+    const uint32_t* __restrict__ gap, FrameCounters* __restrict__ cnt,
+    uint32_t* __restrict__ heavy_list, int tile_cap) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int64_t nvox = (int64_t)min(cnt->n_leaves, tile_cap) * 512;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+       base < nvox; base += stride) {
+    // the wavefront's 64 voxels lie in one tile
+    const int slot = (int)(base >> 9);
+    const Job& J = jobs[__builtin_amdgcn_readfirstlane(slot_job[slot])];
+    const float hit = J.hit, miss = J.miss;
+    const size_t cell = (size_t)base + lane;
+    const uint32_t k = tiles[2 * cell];
+    const uint32_t j = tiles[2 * cell + 1];
+    const bool touched = (k | j) != 0;
+    const bool heavy = j > kHeavyHits;
+    const uint64_t hv = __ballot(heavy);
+    if (hv) {
+      int first = 0;
+      if (lane == (int)__builtin_ctzll(hv))
+        first = atomicAdd(&cnt->n_heavy, (int)__popcll(hv));
+      first = __shfl(first, (int)__builtin_ctzll(hv), 64);
+      if (heavy) heavy_list[first + (int)__popcll(hv & lt)] = (uint32_t)cell;
+    }
+    if (!touched || heavy) continue;
+    float tv;
+    if (j == 0) {
+      tv = fold<M>(0.0f, miss, k);
+    } else {
+      // gaps are loaded 4 at a time so the loads overlap
+      const uint32_t s0 = hit_start[cell];
+      uint32_t used = 0;
+      float acc = 0.0f;
+      for (uint32_t g = 0; g < j; g += 4) {
+        uint32_t gm[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (g + u < j) {
+            if (gm[u] != 0u) {
+              acc = fold_small<M>(acc, miss, gm[u]);
+              used += gm[u];
+            }
+            acc = add<M>(acc, hit);
+          }
+        }
+      }
+      tv = fold<M>(acc, miss, k - used);
+    }
+    tiles[2 * cell] = bits_of(tv);
+    tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+  }
+}
+
+// Temp values of the listed many-hit voxels, one wavefront per voxel.
+template <int M>
+__global__ __launch_bounds__(256) void k_heavy(
+    const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
+    uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
+    const uint32_t* __restrict__ gap, const FrameCounters* __restrict__ cnt,
+    const uint32_t* __restrict__ heavy_list) {
+  const int lane = threadIdx.x & 63;
+  const int nh = cnt->n_heavy;
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nh;
+       q += waves) {
+    const size_t cell = heavy_list[q];
+    const int slot = (int)(cell >> 9);
+    const Job& J = jobs[__builtin_amdgcn_readfirstlane(slot_job[slot])];
+    const uint32_t k = tiles[2 * cell];
+    const uint32_t j = tiles[2 * cell + 1];
+    uint32_t used = 0;
+    const float acc =
+        wave_hit_sequence<M>(gap + hit_start[cell], j, J.miss, J.hit, &used);
+    const float tv = fold<M>(acc, J.miss, k - used);
+    if (lane == 0) {
+      tiles[2 * cell] = bits_of(tv);
+      tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+    }
+  }
+}
+
+// Folds the temp values into each tile's map leaf (Handler.cpp:88-122):
+// map += temp for nonzero temp values, clamp, activity thresholds.  One
+// workgroup per tile; clears the tile for the next frame.
+template <int M>
+__global__ __launch_bounds__(512) void k_apply(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     int32_t* __restrict__ leaf_slot, const int32_t* __restrict__ slot_leaf,
     uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
-    const uint32_t* __restrict__ hit_start, const uint32_t* __restrict__ gap,
     const int32_t* __restrict__ map_idx, const int32_t* __restrict__ map_new,
     const MapView* __restrict__ maps, MapView single, int tile_cap,
     FrameCounters* __restrict__ cnt, uint32_t* __restrict__ slot_stats) {
   __shared__ uint32_t s_part[8][3];
   __shared__ int s_new;
-  __shared__ int s_wcount[8];
-  __shared__ int s_list[512];
-  __shared__ float s_tv[512];
   const int li = threadIdx.x;
   const int lane = li & 63, wave = li >> 6;
   const int nleaves = min(cnt->n_leaves, tile_cap);
   for (int slot = blockIdx.x; slot < nleaves; slot += gridDim.x) {
     const int jb = __builtin_amdgcn_readfirstlane(slot_job[slot]);
     const Job& J = jobs[jb];
-    const float hit = J.hit, miss = J.miss;
     const MapView m = maps ? maps[J.map] : single;
     const int c = slot_leaf[slot];
-    int ox, oy, oz;
-    slot_origin(J, c - (int)J.cube_off, ox, oy, oz);
     const size_t cell = (size_t)slot * 512 + li;
-    const uint32_t k = tiles[2 * cell];      // misses
-    const uint32_t j = tiles[2 * cell + 1];  // hits
-    const bool touched = (k | j) != 0;
+    const uint32_t t0 = tiles[2 * cell];
+    const uint32_t t1 = tiles[2 * cell + 1];
     const int known = map_idx[slot];
     float old_v = 0.0f;
     uint32_t old_w = 0;
-    if (known >= 0) {  // issue the map loads before the fold
+    if (known >= 0) {
       old_v = m.values[(size_t)known * 512 + li];
       old_w = m.masks[(size_t)known * kMaskWords + (li >> 5)];
     }
-    float tv = 0.0f;
-    const bool heavy = j > kHeavyHits;
-    if (touched && !heavy) {
-      if (j == 0) {
-        tv = fold<M>(0.0f, miss, k);
-      } else {
-        // gaps are loaded 4 at a time so the loads overlap
-        const uint32_t s0 = hit_start[cell];
-        uint32_t used = 0;
-        float acc = 0.0f;
-        for (uint32_t g = 0; g < j; g += 4) {
-          uint32_t gm[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (g + u < j) {
-              if (gm[u] != 0u) {
-                acc = fold_small<M>(acc, miss, gm[u]);
-                used += gm[u];
-              }
-              acc = add<M>(acc, hit);
-            }
-          }
-        }
-        tv = fold<M>(acc, miss, k - used);
-      }
-    }
-    // Voxels with many hits (a surface close to the sensor) are summed by a
-    // whole wavefront each, so one long serial sequence does not hold up
-    // the leaf.
-    if (__syncthreads_or(heavy)) {
-      const uint64_t hb = __ballot(heavy);
-      if (lane == 0) s_wcount[wave] = (int)__popcll(hb);
-      __syncthreads();
-      int before = 0, nheavy = 0;
-      for (int w = 0; w < 8; ++w) {
-        before += w < wave ? s_wcount[w] : 0;
-        nheavy += s_wcount[w];
-      }
-      if (heavy) {
-        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        s_list[before + (int)__popcll(hb & lt)] = li;
-      }
-      __syncthreads();
-      for (int q = wave; q < nheavy; q += 8) {
-        const int v = s_list[q];
-        const size_t cv = (size_t)slot * 512 + v;
-        const uint32_t kv = tiles[2 * cv];
-        const uint32_t jv = tiles[2 * cv + 1];
-        uint32_t used = 0;
-        const float acc =
-            wave_hit_sequence<M>(gap + hit_start[cv], jv, miss, hit, &used);
-        const float t = fold<M>(acc, miss, kv - used);
-        if (lane == 0) s_tv[v] = t;
-      }
-      __syncthreads();
-      if (heavy) tv = s_tv[li];
-    }
+    const bool touched = (t1 & kTouchedBit) != 0;
+    const uint32_t nhits = t1 & kHitsMask;
+    const float tv = touched ? float_of(t0) : 0.0f;
     const bool nz = touched && tv != 0.0f;
     const int any = __syncthreads_or(nz);
     int idx = known;
@@ -823,6 +873,8 @@ __global__ __launch_bounds__(512, 4) void k_update(
         const int r = map_new[slot];
         bool ok = false;
         if (r >= 0) {
+          int ox, oy, oz;
+          slot_origin(J, c - (int)J.cube_off, ox, oy, oz);
           ok = any && map_insert(m, ox, oy, oz, r);
           if (any && !ok) atomicAdd(&cnt->overflow, 1);
           if (!ok) {  // the reserved pool leaf stays a (zero) hole
@@ -835,30 +887,28 @@ __global__ __launch_bounds__(512, 4) void k_update(
       __syncthreads();
       idx = s_new;
     }
-    if (any) {
-      if (idx >= 0) {
-        float* mv = m.values + (size_t)idx * 512;
-        uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
-        bool act = (old_w >> (li & 31)) & 1u;
-        if (nz) {
-          float v = add<M>(old_v, tv);
-          if (v > m.clamp_max) {
-            v = m.clamp_max;
-          } else if (v < m.clamp_min) {
-            v = m.clamp_min;
-          }
-          if (v > m.hit_thr) {
-            act = true;
-          } else if (v < m.miss_thr) {
-            act = false;
-          }
-          mv[li] = v;
+    if (any && idx >= 0) {
+      float* mv = m.values + (size_t)idx * 512;
+      uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
+      bool act = (old_w >> (li & 31)) & 1u;
+      if (nz) {
+        float v = add<M>(old_v, tv);
+        if (v > m.clamp_max) {
+          v = m.clamp_max;
+        } else if (v < m.clamp_min) {
+          v = m.clamp_min;
         }
-        const uint64_t b = __ballot(act);
-        if (lane == 0) {
-          mm[2 * wave] = (uint32_t)b;
-          mm[2 * wave + 1] = (uint32_t)(b >> 32);
+        if (v > m.hit_thr) {
+          act = true;
+        } else if (v < m.miss_thr) {
+          act = false;
         }
+        mv[li] = v;
+      }
+      const uint64_t b = __ballot(act);
+      if (lane == 0) {
+        mm[2 * wave] = (uint32_t)b;
+        mm[2 * wave + 1] = (uint32_t)(b >> 32);
       }
     }
     if (touched) {
@@ -869,8 +919,8 @@ __global__ __launch_bounds__(512, 4) void k_update(
     if (li == 0) leaf_slot[c] = -1;
     // per-slot statistics: voxels touched, with hits, with hits and misses
     const uint64_t bt = __ballot(touched);
-    const uint64_t bh = __ballot(j != 0);
-    const uint64_t bm = __ballot(j != 0 && k != 0);
+    const uint64_t bh = __ballot(nhits != 0);
+    const uint64_t bm = __ballot(nhits != 0 && (t1 & kMixedBit));
     if (lane == 0) {
       s_part[wave][0] = (uint32_t)__popcll(bt);
       s_part[wave][1] = (uint32_t)__popcll(bh);
@@ -1050,19 +1100,31 @@ hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
   return hipGetLastError();
 }
 
+template <int M, int S>
+void accumulate_as(const Pipeline& p, const Workspace& b, int sorted_buf,
+                   const ItemQueue& q, int grid, hipStream_t s) {
+  k_accumulate<M, S><<<grid, kCastThreads, 0, s>>>(
+      b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
+      b.vals[sorted_buf], b.gap, b.counters);
+}
+
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
                              int sorted_buf, hipStream_t s) {
   if (p.nitems <= 0) return hipSuccess;
   int grid = 0;
   const ItemQueue q = cast_queue(p, b, 1, &grid);
-  if (p.rounding == kDown)
-    k_accumulate<kDown><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
-        b.vals[sorted_buf], b.gap, b.counters);
-  else
-    k_accumulate<kNearest><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
-        b.vals[sorted_buf], b.gap, b.counters);
+  const bool small = p.tuning.lds_slots == 2048;
+  if (p.rounding == kDown) {
+    if (small)
+      accumulate_as<kDown, 11>(p, b, sorted_buf, q, grid, s);
+    else
+      accumulate_as<kDown, 12>(p, b, sorted_buf, q, grid, s);
+  } else {
+    if (small)
+      accumulate_as<kNearest, 11>(p, b, sorted_buf, q, grid, s);
+    else
+      accumulate_as<kNearest, 12>(p, b, sorted_buf, q, grid, s);
+  }
   return hipGetLastError();
 }
 
@@ -1072,17 +1134,31 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
                                    single, p.tile_cap, b.counters, b.map_idx,
                                    b.map_new);
-  const int grid = kUpdateGrid;
-  if (p.rounding == kDown)
-    k_update<kDown><<<grid, 512, 0, s>>>(
+  const int tgrid = p.cu_count * 8;
+  const int hgrid = p.cu_count * 2;
+  if (p.rounding == kDown) {
+    k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
+                                        b.hit_start, b.gap, b.counters,
+                                        b.heavy, p.tile_cap);
+    k_heavy<kDown><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
+                                         b.hit_start, b.gap, b.counters,
+                                         b.heavy);
+    k_apply<kDown><<<kUpdateGrid, 512, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
-        b.hit_start, b.gap, b.map_idx, b.map_new, map_table, single,
-        p.tile_cap, b.counters, b.slot_stats);
-  else
-    k_update<kNearest><<<grid, 512, 0, s>>>(
+        b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
+        b.slot_stats);
+  } else {
+    k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
+                                           b.hit_start, b.gap, b.counters,
+                                           b.heavy, p.tile_cap);
+    k_heavy<kNearest><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
+                                            b.hit_start, b.gap, b.counters,
+                                            b.heavy);
+    k_apply<kNearest><<<kUpdateGrid, 512, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
-        b.hit_start, b.gap, b.map_idx, b.map_new, map_table, single,
-        p.tile_cap, b.counters, b.slot_stats);
+        b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
+        b.slot_stats);
+  }
   return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@ struct FrameCounters {
   int32_t n_filter_rays;    // rays emitted by the voxel filter
   int32_t n_filter_points;  // points binned by the voxel filter
   int32_t queue[4];         // work-queue heads of the persistent kernels
+  int32_t n_heavy;          // voxels with many hits, listed for k_heavy
 };
 
 // Statistics of one job, reduced on demand from per-item and per-slot
@@ -115,6 +116,8 @@ struct Tuning {
                       // draining a queue in job order; 2: same, costliest
                       // items first
   int grid_mult = 4;  // persistent workgroups per compute unit
+  int lds_slots = 4096;  // LDS aggregation table of the accumulate pass:
+                         // 4096 or 2048 entries
 };
 
 // Device workspace shared by all jobs of a launch sequence.
@@ -133,6 +136,8 @@ struct Workspace {
   uint32_t* keys[2];    // [ray_cap]  radix sort buffers
   uint32_t* vals[2];
   uint32_t* gap;        // [ray_cap]  misses before each sorted hit
+  uint32_t* heavy;      // [ray_cap / kHeavyHits + 1]  cells of voxels with
+                        //             more than kHeavyHits hits
   uint32_t* radix_hist; // radix_hist_words(ray_cap)
   int32_t* scan_blocks; // [cube_cap / kScanChunk + 2]
   int32_t* scan_job;    // [cube_cap / kScanChunk + 2]  job of each chunk
