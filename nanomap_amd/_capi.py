@@ -1,6 +1,6 @@
 """ctypes binding of the C ABI in include/nanomap_amd.h,
-include/nanomap_amd_batch.h, include/nanomap_amd_sim.h and
-include/nanomap_amd_scene.h.
+include/nanomap_amd_batch.h, include/nanomap_amd_sim.h,
+include/nanomap_amd_soft.h and include/nanomap_amd_scene.h.
 
 The shared library is built in-tree (nanomap_amd/libnanomap_amd.so).  If it
 is missing, every entry point raises: there is no CPU fallback.
@@ -84,6 +84,18 @@ class BatchTuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SoftGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("origin", ctypes.c_double * 3),
+                ("res", ctypes.c_float), ("background", ctypes.c_float)]
+
+
+class SoftParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
+                ("step", ctypes.c_float), ("n_samples", ctypes.c_int)]
+
+
 class SimSensor(ctypes.Structure):
     _fields_ = [("position", ctypes.c_float * 3),
                 ("rotation", ctypes.c_float * 9), ("grid_res", ctypes.c_float),
@@ -145,6 +157,16 @@ SIGNATURES = {
     "nm_batch_set_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
     "nm_batch_get_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
     "nm_batch_set_graph": (_I, [_P, _I]),
+    "nm_soft_forward": (_I, [ctypes.POINTER(SoftGrid), _P, _P, _I,
+                             ctypes.POINTER(SoftParams), _P, _P, _P]),
+    "nm_soft_backward": (_I, [ctypes.POINTER(SoftGrid), _P, _P, _I,
+                              ctypes.POINTER(SoftParams), _P, _P, _P, _P, _P,
+                              _P]),
+    "nm_soft_forward_f64": (_I, [ctypes.POINTER(SoftGrid), _P, _P, _I,
+                                 ctypes.POINTER(SoftParams), _P, _P, _P]),
+    "nm_soft_backward_f64": (_I, [ctypes.POINTER(SoftGrid), _P, _P, _I,
+                                  ctypes.POINTER(SoftParams), _P, _P, _P, _P,
+                                  _P]),
     "nm_scene_create": (_I, [ctypes.c_uint64, _PD, _I, _D, _D, _I, _D, _D,
                              ctypes.POINTER(_P)]),
     "nm_scene_destroy": (_I, [_P]),

@@ -10,6 +10,9 @@ cfg4: 32 agents in a 100x100x10 m room (1200 boxes + 800 cylinders), each
       848x480, 1280x720} (87x58 deg for 4:3, 90x60 deg for 16:9), range
       uniform 3-20 m, voxel size from {0.05, 0.1, 0.2} m, all from the seed;
       depth noise and dropouts as cfg2.
+cfg5: the cfg1 world mapped at 0.05 m from a 48-frame random walk of
+      320x240 frames (87x58 deg, 8 m range); 16 soft-march cameras at every
+      third pose of the walk (320x240, 8 m: 160 samples 0.05 m apart).
 Log-odds hit +0.85, miss -0.4, clamp [-2.0, 3.5], thresholds 0/0.
 """
 import numpy as np
@@ -107,3 +110,28 @@ def agent_frame(nm, a, rounding):
     return nm.frame_from_pose(a["pos"], a["quat"], nm.DEFAULT_F, a["res"],
                               a["range"], HIT, MISS, rounding, a["width"],
                               a["height"])
+
+
+def cfg5(nm, n_walk=48, seed=5):
+    """(scene, camera, positions, quaternions, clouds) of the cfg5 walk."""
+    scene = nm.Scene(1, (20.0, 20.0, 5.0), 40, 0.3, 2.0)
+    pos, quat = scene.walk(seed, n_walk, 0.1, 2.0, 15.0, 0.3)
+    cam = nm.camera(320, 240, 87.0, 58.0, 8.0, noise_k=0.002, dropout=0.0)
+    clouds = [scene.render(cam, pos[k], quat[k], seed * 1000 + k)
+              for k in range(n_walk)]
+    return scene, cam, pos, quat, clouds
+
+
+def dense_grid(origins, values, pad=2):
+    """Dense [nx, ny, nz] float32 log-odds over the leaves' bounding box
+    (plus pad voxels), and the index of its voxel (0, 0, 0)."""
+    o = np.asarray(origins, np.int64)
+    lo = o.min(0) - pad
+    hi = o.max(0) + 8 + pad
+    shape = tuple(int(x) for x in (hi - lo))
+    g = np.zeros(shape, np.float32)
+    li = np.arange(512)
+    off = np.stack([li >> 6, (li >> 3) & 7, li & 7], 1)
+    ijk = (o[:, None, :] + off[None, :, :] - lo).reshape(-1, 3)
+    g[ijk[:, 0], ijk[:, 1], ijk[:, 2]] = np.asarray(values, np.float32).reshape(-1)
+    return g, lo

@@ -20,6 +20,7 @@ _PI32 = ctypes.POINTER(ctypes.c_int32)
 _PU32 = ctypes.POINTER(ctypes.c_uint32)
 _PF = ctypes.POINTER(ctypes.c_float)
 _PD = ctypes.POINTER(ctypes.c_double)
+_D = ctypes.c_double
 
 SIGNATURES = {
     "or_temp_create": (_P, []),
@@ -45,6 +46,10 @@ SIGNATURES = {
     "or_simgrid_active_count": (ctypes.c_int64, [_P]),
     "or_sim_cast": (_I, [_P, _PF, _PF, _F, _I, _I, _F, _F, _F, _F, _I, _PI32,
                          _PF, _PF]),
+    "or_soft_forward": (None, [_PD, _I, _I, _I, _PD, _D, _D, _PD, _I, _I, _I,
+                               _D, _D, _D, _I, _PD]),
+    "or_soft_backward": (None, [_PD, _I, _I, _I, _PD, _D, _D, _PD, _I, _I,
+                                _I, _D, _D, _D, _I, _PD, _PD, _PD]),
 }
 
 _lib = None
@@ -232,3 +237,63 @@ class SimGrid:
                               int(hierarchical), _p(ijk, ctypes.c_int32),
                               _p(dist, ctypes.c_float), _p(xyz, ctypes.c_float))
         return ijk, dist, xyz, n
+
+
+class SoftScene:
+    """Dense FP64 log-odds grid and camera parameters of the soft march
+    (the model of include/nanomap_amd_soft.h, restated in FP64)."""
+
+    def __init__(self, grid, origin, res, background, width, height, hfov,
+                 vfov, step, n_samples):
+        self.grid = np.ascontiguousarray(grid, np.float64)
+        self.origin = np.ascontiguousarray(origin, np.float64)
+        self.res, self.bg = float(res), float(background)
+        self.width, self.height = int(width), int(height)
+        self.hfov, self.vfov = float(hfov), float(vfov)
+        self.step, self.ns = float(step), int(n_samples)
+
+    def _args(self, cams):
+        c = np.ascontiguousarray(cams, np.float64).reshape(-1, 12)
+        nx, ny, nz = self.grid.shape
+        return c, [_p(self.grid, ctypes.c_double), nx, ny, nz,
+                   _p(self.origin, ctypes.c_double), self.res, self.bg,
+                   _p(c, ctypes.c_double), c.shape[0], self.width,
+                   self.height, self.hfov, self.vfov, self.step, self.ns]
+
+    def forward(self, cams):
+        """Expected depth [n_cams, height, width]."""
+        c, args = self._args(cams)
+        out = np.zeros((c.shape[0], self.height, self.width), np.float64)
+        lib().or_soft_forward(*args, _p(out, ctypes.c_double))
+        return out
+
+    def backward(self, cams, grad_depth, want_grid=True):
+        """(dL/dxi [n_cams, 6] as (rho, omega), dL/dgrid or None)."""
+        c, args = self._args(cams)
+        g = np.ascontiguousarray(grad_depth, np.float64)
+        gp = np.zeros((c.shape[0], 6), np.float64)
+        gg = np.zeros_like(self.grid) if want_grid else None
+        lib().or_soft_backward(*args, _p(g, ctypes.c_double),
+                               _p(gp, ctypes.c_double),
+                               _p(gg, ctypes.c_double) if want_grid else None)
+        return gp, gg
+
+
+def perturb_camera(cam, xi):
+    """Camera (12 doubles) moved by the world-frame perturbation
+    xi = (rho, omega): R' = exp(omega^) R, p' = p + rho (rotation about the
+    camera centre), in FP64 (Rodrigues)."""
+    cam = np.asarray(cam, np.float64).reshape(12)
+    R = cam[:9].reshape(3, 3)
+    w = np.asarray(xi[3:], np.float64)
+    th = np.linalg.norm(w)
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if th < 1e-300:
+        E = np.eye(3)
+    else:
+        E = (np.eye(3) + np.sin(th) / th * K
+             + (1 - np.cos(th)) / th ** 2 * K @ K)
+    out = np.empty(12)
+    out[:9] = (E @ R).reshape(9)
+    out[9:] = cam[9:] + np.asarray(xi[:3], np.float64)
+    return out

@@ -103,6 +103,21 @@ int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float vfov_deg, float min_range, float max_range,
                 int hierarchical, int32_t* ijk, float* dist, float* xyz);
 
+/* ---- soft expected-depth march (include/nanomap_amd_soft.h), FP64 ------
+ * grid: dense [nx][ny][nz]; cams: n_cams x 12 doubles (row-major optical ->
+ * world rotation, then position).  Outputs accumulate (+=). */
+void or_soft_forward(const double* grid, int nx, int ny, int nz,
+                     const double origin[3], double res, double background,
+                     const double* cams, int n_cams, int width, int height,
+                     double hfov_deg, double vfov_deg, double step,
+                     int n_samples, double* depth);
+void or_soft_backward(const double* grid, int nx, int ny, int nz,
+                      const double origin[3], double res, double background,
+                      const double* cams, int n_cams, int width, int height,
+                      double hfov_deg, double vfov_deg, double step,
+                      int n_samples, const double* grad_depth,
+                      double* grad_pose, double* grad_logodds);
+
 #ifdef __cplusplus
 }
 #endif
