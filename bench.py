@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--rounding", choices=["down", "nearest"], default="down")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed passes of the 32-agent batch (0 = skip)")
+    ap.add_argument("--soft-steps", type=int, default=10,
+                    help="timed passes of the soft march (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed steps, run one fill and one "
                          "elementwise copy of this many MiB (the known-size "
@@ -136,6 +138,112 @@ def bench_batch(nm, W, torch, steps, rounding):
         "speedup": plain_ms / batch_ms,
         "tuning": tuning,
         "tuner_s": tune_s,
+        "steps": steps,
+    }
+
+
+def bench_soft(nm, W, torch, steps):
+    """Forward + adjoint of the differentiable soft march for 16 cameras of
+    320x240 (8 m, 160 samples at 0.05 m) over the config-1 world mapped at
+    0.05 m: one launch each for all cameras, against one launch pair per
+    camera; plus 20 Gauss-Newton pose-refinement steps."""
+    import numpy as np
+    from nanomap_amd import soft
+    res = 0.05
+    _, cam, pos, quat, clouds = W.cfg5(nm)
+    ctx = nm.Context(cam.width * cam.height, 8.0 / res + 1)
+    m = nm.Map(res, *W.CLAMP, leaf_capacity=1 << 19)
+    frames = []
+    for k in range(len(clouds)):
+        f = W.frame(nm, pos[k], quat[k], res, 8.0, nm.ROUND_DOWN)
+        ctx.submit(m, f, clouds[k])
+        frames.append(f)
+    o, v, _ = m.download()
+    grid, lo = W.dense_grid(o, v)
+    sel = list(range(0, 48, 3))[:16]
+    R = torch.tensor(np.stack([np.array(list(frames[k].rotation),
+                                        np.float64).reshape(3, 3)
+                               for k in sel])).cuda()
+    p = torch.tensor(np.stack([np.asarray(pos[k], np.float64)
+                               for k in sel])).cuda()
+    march = soft.SoftMarch(grid.shape, lo * res, res, 320, 240, 87.0, 58.0,
+                           res, 160)
+    g32 = torch.from_numpy(grid).cuda()
+    cams = soft.cameras(R, p)
+    n = cams.shape[0]
+    gout = torch.ones((n, 240, 320), dtype=torch.float32, device="cuda")
+    gp = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    import ctypes
+    from nanomap_amd._capi import check, lib
+    depth = torch.empty((n, 240, 320), dtype=torch.float32, device="cuda")
+    logt = torch.empty((n, 240, 320), dtype=torch.float64, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def fwd_bwd(lo_c, hi_c):
+        # cameras [lo_c, hi_c) in one forward and one adjoint launch
+        c = cams[lo_c:hi_c]
+        k = hi_c - lo_c
+        off = lo_c * 240 * 320
+        check(lib().nm_soft_forward(ctypes.byref(march.grid),
+                                    ctypes.c_void_p(g32.data_ptr()),
+                                    ctypes.c_void_p(c.data_ptr()), k,
+                                    ctypes.byref(march.params),
+                                    ctypes.c_void_p(depth.data_ptr() + 4 * off),
+                                    ctypes.c_void_p(logt.data_ptr() + 8 * off), st))
+        check(lib().nm_soft_backward(ctypes.byref(march.grid),
+                                     ctypes.c_void_p(g32.data_ptr()),
+                                     ctypes.c_void_p(c.data_ptr()), k,
+                                     ctypes.byref(march.params),
+                                     ctypes.c_void_p(logt.data_ptr() + 8 * off),
+                                     ctypes.c_void_p(gout.data_ptr() + 4 * off),
+                                     ctypes.c_void_p(gp.data_ptr() + 8 * 6 * lo_c),
+                                     None, None, st))
+
+    def timed(fn):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    batched_ms = timed(lambda: fwd_bwd(0, n))
+    per_ms = timed(lambda: [fwd_bwd(i, i + 1) for i in range(n)])
+    rays = n * 240 * 320
+    # pose refinement from perturbations of up to 0.1 m and 3 deg
+    target = march.forward(g32, cams)
+    gen = torch.Generator().manual_seed(11)
+    dt = (torch.rand((n, 3), generator=gen, dtype=torch.float64) * 2 - 1) * 0.1
+    axis = torch.randn((n, 3), generator=gen, dtype=torch.float64)
+    axis = axis / axis.norm(dim=1, keepdim=True)
+    ang = (torch.rand((n, 1), generator=gen, dtype=torch.float64) * 2 - 1) * np.radians(3)
+    R0 = soft.so3_exp((axis * ang).cuda()) @ R
+    p0 = p + dt.cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Rf, pf, hist = soft.refine_poses(march, g32, R0, p0, target, iters=20)
+    torch.cuda.synchronize()
+    refine_s = time.perf_counter() - t0
+    t_err = (pf - p).norm(dim=1)
+    cosang = ((Rf.transpose(1, 2) @ R).diagonal(dim1=1, dim2=2).sum(1) - 1) / 2
+    r_err = torch.rad2deg(torch.arccos(cosang.clamp(-1, 1)))
+    return {
+        "workload": "16 cameras 320x240 (87x58 deg), 8 m, 160 samples at "
+                    "0.05 m, quadratic B-spline over the config-1 world "
+                    "mapped at 0.05 m (%dx%dx%d grid); forward + adjoint "
+                    "pose gradients" % grid.shape,
+        "rays": rays,
+        "batched_ms": batched_ms,
+        "per_frame_ms": per_ms,
+        "batched_mrays_per_s": rays / batched_ms / 1e3,
+        "per_frame_mrays_per_s": rays / per_ms / 1e3,
+        "speedup": per_ms / batched_ms,
+        "refine_20_steps_s": refine_s,
+        "refine_translation_err_m_median": float(t_err.median()),
+        "refine_rotation_err_deg_median": float(r_err.median()),
         "steps": steps,
     }
 
@@ -341,6 +449,9 @@ def main():
     if args.batch_steps > 0:
         out["batched"] = bench_batch(nm, W, torch, args.batch_steps, rounding)
         print("batched done", file=sys.stderr, flush=True)
+    if args.soft_steps > 0:
+        out["soft_march"] = bench_soft(nm, W, torch, args.soft_steps)
+        print("soft march done", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(params, clouds,
                                            min(args.cpu_frames, args.frames))

@@ -48,6 +48,17 @@ __device__ __forceinline__ double sigmoid_(double l) {
 }
 
 template <typename T>
+__device__ __forceinline__ T exp_t(T x);
+template <>
+__device__ __forceinline__ float exp_t(float x) {
+  return expf(x);
+}
+template <>
+__device__ __forceinline__ double exp_t(double x) {
+  return exp(x);
+}
+
+template <typename T>
 __device__ __forceinline__ T floor_t(T x);
 template <>
 __device__ __forceinline__ float floor_t(float x) {
@@ -146,6 +157,27 @@ __device__ __forceinline__ T spline(const SoftArgs<T>& A, const Taps<T>& tp,
   return v;
 }
 
+// Value only (the forward pass).
+template <typename T>
+__device__ __forceinline__ T spline_value(const SoftArgs<T>& A,
+                                          const Taps<T>& tp) {
+  const bool in = taps_inside(A, tp);
+  T v = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      T r = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        r += tp.w[2][c] * coef(A, tp.b[0] + a - 1, tp.b[1] + b - 1,
+                               tp.b[2] + c - 1, in);
+      v += tp.w[0][a] * tp.w[1][b] * r;
+    }
+  }
+  return v;
+}
+
 // Ray of thread `tid` in tile `t` of camera `c`: eye and direction in grid
 // voxel units (e, dv) and the world direction (dir).  False outside the
 // image.
@@ -202,8 +234,7 @@ __global__ __launch_bounds__(256) void k_soft_forward(SoftArgs<T> A,
     const T q[3] = {e[0] + dv[0] * d, e[1] + dv[1] * d, e[2] + dv[2] * d};
     Taps<T> tp;
     taps_at(q, tp);
-    T g[3];
-    const T l = spline(A, tp, g);
+    const T l = spline_value(A, tp);
     const T p = sigmoid_(l);
     const T w = p * tr;
     D += w * d;
@@ -222,7 +253,7 @@ __device__ __forceinline__ void atomic_add_t(double* p, double v) {
   atomicAdd(p, v);
 }
 
-template <typename T, typename G>
+template <typename T, typename G, bool GRID>
 __global__ __launch_bounds__(256) void k_soft_backward(
     SoftArgs<T> A, const double* __restrict__ logt,
     const T* __restrict__ gdepth, double* __restrict__ gpose,
@@ -251,7 +282,7 @@ __global__ __launch_bounds__(256) void k_soft_backward(
       const T p = sigmoid_(l);
       const T om = T(1) - p;
       lt -= (double)log1m_sigmoid(l);
-      const T tk = (T)exp(lt);
+      const T tk = exp_t((T)lt);
       const T dDdl = tk * (d - E) * p * om;
       E = p * d + om * E;
       if (dDdl == T(0)) continue;
@@ -264,7 +295,7 @@ __global__ __launch_bounds__(256) void k_soft_backward(
       jw[0] += dDdl * (ry * gz - rz * gy);
       jw[1] += dDdl * (rz * gx - rx * gz);
       jw[2] += dDdl * (rx * gy - ry * gx);
-      if (ggrid && gout != T(0)) {
+      if (GRID && gout != T(0)) {
         const T s = gout * dDdl;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -381,10 +412,16 @@ int nm_soft_backward(const nm_soft_grid* g, const float* logodds,
     nm::set_error("nm_soft_backward: bad argument");
     return NM_EINVAL;
   }
-  nm::k_soft_backward<float, float>
-      <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
-          A, log_transmittance, grad_depth, grad_pose, grad_ray_pose,
-          grad_logodds);
+  if (grad_logodds)
+    nm::k_soft_backward<float, float, true>
+        <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
+            A, log_transmittance, grad_depth, grad_pose, grad_ray_pose,
+            grad_logodds);
+  else
+    nm::k_soft_backward<float, float, false>
+        <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
+            A, log_transmittance, grad_depth, grad_pose, grad_ray_pose,
+            nullptr);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     nm::set_error(std::string("nm_soft_backward: ") + hipGetErrorString(e));
@@ -424,9 +461,15 @@ int nm_soft_backward_f64(const nm_soft_grid* g, const double* logodds,
     nm::set_error("nm_soft_backward_f64: bad argument");
     return NM_EINVAL;
   }
-  nm::k_soft_backward<double, double>
-      <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
-          A, log_transmittance, grad_depth, grad_pose, nullptr, grad_logodds);
+  if (grad_logodds)
+    nm::k_soft_backward<double, double, true>
+        <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
+            A, log_transmittance, grad_depth, grad_pose, nullptr,
+            grad_logodds);
+  else
+    nm::k_soft_backward<double, double, false>
+        <<<nm::soft_grid(p, n), 256, 0, (hipStream_t)stream>>>(
+            A, log_transmittance, grad_depth, grad_pose, nullptr, nullptr);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     nm::set_error(std::string("nm_soft_backward_f64: ") + hipGetErrorString(e));
