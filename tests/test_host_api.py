@@ -1,0 +1,197 @@
+"""The C++ host API (nanomap_amd/host/nanomap/...: Manager, map::Map,
+config::Config, sensor::SensorData, Pose), driven by tests/cpp/test_host.
+
+CPU: config and sensor files in the reference's text formats
+(src/nanomap/config/Config.cpp) parse to the expected values.
+GPU: Manager::insertPointCloud over several frames builds a map equal bit
+for bit to the oracle's Handler::processPointCloudCPU on the same inputs,
+with and without the voxel filter.
+"""
+import math
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from nanomap_amd import build as nb
+
+DRIVER = nb.HOST_TEST
+
+FRUSTUM = """#frustumconfig
+Name:
+{name}
+Id:
+{id}
+HRes:
+{w}
+VRes:
+{h}
+VFOV:
+{vfov}
+Rate:
+30
+MaxRange:
+{rmax}
+MinRange:
+0.1
+ProbHit:
+{ph}
+ProbMiss:
+{pm}
+FrameTransform:
+0.0 -1.0 0.0
+0.0 0.0 1.0
+1.0 0.0 0.0
+#endconfig
+"""
+
+LASER = """#laserconfig
+Name:
+lidar
+Id:
+2
+AngularHRes:
+0.4
+AngularVRes:
+2.0
+HFOV:
+360
+VFOV:
+30
+Rate:
+10
+MaxRange:
+30
+MinRange:
+0.5
+ProbHit:
+0.8
+ProbMiss:
+0.35
+FrameTransform:
+1.0 0.0 0.0
+0.0 1.0 0.0
+0.0 0.0 1.0
+#endconfig
+"""
+
+
+def _write_config(tmp, res, filter_type, sensors):
+    paths = []
+    for i, text in enumerate(sensors):
+        p = os.path.join(tmp, "sensor%d.txt" % i)
+        with open(p, "w") as f:
+            f.write(text)
+        paths.append(p)
+    cfg = os.path.join(tmp, "config.txt")
+    with open(cfg, "w") as f:
+        f.write("#config\nMappingRes:\n%s\nProbHitThres:\n0.5\nProbMissThres:\n"
+                "0.5\nGridRes:\n%s\nNodeEdge:\n8\nFrustumAllocationFactor:\n0.4\n"
+                "FilterType:\n%d\nUpdateType:\n0\nPrecisionType:\n0\n"
+                "PublishSensor:\n1\n" % (res, res, filter_type))
+        for p in paths:
+            f.write("Sensor:\n%s\n" % p)
+        f.write("#endconfig\n")
+    return cfg
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(DRIVER):
+        nb.build_host_test()
+    return DRIVER
+
+
+def _logit(p):
+    return np.float32(math.log(float(np.float32(p))) - math.log(1.0 - float(np.float32(p))))
+
+
+def test_config_and_sensor_files_parse(driver, tmp_path):
+    cfg = _write_config(str(tmp_path), 0.05, 1, [
+        FRUSTUM.format(name="cam", id=1000, w=640, h=480, vfov=43, rmax=10,
+                       ph=0.95, pm=0.3), LASER])
+    out = subprocess.run([driver, "parse", cfg], capture_output=True,
+                         text=True, check=True).stdout.split("\n")
+    kv = dict(l.split(" ", 1) for l in out if l and not l.startswith("sensor "))
+    assert float(kv["MappingRes"]) == np.float32(0.05)
+    assert kv["FilterType"] == "1" and kv["NodeEdge"] == "8"
+    assert kv["sensors"] == "2" and kv["lookup"] == "1"
+    sens = [l.split() for l in out if l.startswith("sensor ")]
+    cam = sens[0]
+    assert cam[1] == "cam" and cam[3] == "1000" and cam[5] == "0"
+    assert cam[7] == "640x480"
+    # logit(p) in FP64, rounded to FP32
+    assert np.float32(cam[19]) == _logit(0.95) and np.float32(cam[20]) == _logit(0.3)
+    # FrameTransform: column i = file line i (Config.cpp:249-256)
+    F = np.array([float(x) for x in cam[22:31]]).reshape(3, 3)
+    assert np.array_equal(F, [[0, 0, 1], [-1, 0, 0], [0, 1, 0]])
+    lidar = sens[1]
+    # beams = (int)(FOV / angular resolution + 1) (LaserData.h:23-24)
+    assert lidar[5] == "1" and lidar[7] == "%dx%d" % (int(360 / 0.4 + 1), int(30 / 2.0 + 1))
+    assert int(kv["maxPCLSize"]) == max(640 * 480, int(360 / 0.4 + 1) * int(30 / 2.0 + 1))
+
+
+def test_bad_config_is_reported(driver, tmp_path):
+    p = os.path.join(str(tmp_path), "bad.txt")
+    with open(p, "w") as f:
+        f.write("#notaconfig\n")
+    r = subprocess.run([driver, "parse", p], capture_output=True, text=True)
+    assert r.returncode == 1 and "#config" in r.stderr
+
+
+def _read_map(path):
+    with open(path, "rb") as f:
+        n = struct.unpack("<q", f.read(8))[0]
+        o = np.frombuffer(f.read(n * 12), np.int32).reshape(n, 3)
+        v = np.frombuffer(f.read(n * 2048), np.float32).reshape(n, 512)
+        m = np.frombuffer(f.read(n * 64), np.uint32).reshape(n, 16)
+    order = np.lexsort((o[:, 2], o[:, 1], o[:, 0]))
+    return o[order], v[order], m[order]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("filter_type", [0, 1])
+def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type):
+    nm = gpu
+    res, rmax, w, h = 0.1, 5.0, 160, 120
+    cfg = _write_config(str(tmp_path), res, filter_type, [
+        FRUSTUM.format(name="cam", id=1, w=w, h=h, vfov=45, rmax=rmax,
+                       ph=0.7, pm=0.4)])
+    scene = nm.Scene(1, (20.0, 20.0, 5.0), 40, 0.3, 2.0)
+    pos, quat = scene.walk(8, 3, 0.1, 2.0, 15.0, 0.3)
+    cam = nm.camera(w, h, 60.0, 45.0, rmax, noise_k=0.002, dropout=0.05)
+    args = []
+    # map::Map's clamps, logit(0.12) and logit(0.97) (OccupancyMap.h:41-42)
+    omap = oracle.Map(float(np.float32(math.log(0.12) - math.log(1 - 0.12))),
+                      float(np.float32(math.log(0.97) - math.log(1 - 0.97))),
+                      0.0, 0.0)
+    for k in range(3):
+        cloud = scene.render(cam, pos[k], quat[k], 100 + k)
+        path = os.path.join(str(tmp_path), "cloud%d.bin" % k)
+        cloud.astype(np.float32).tofile(path)
+        p, q = pos[k], quat[k]
+        args.append("%s,%r,%r,%r,%r,%r,%r,%r" % (path, *[float(x) for x in p],
+                                                 *[float(x) for x in q]))
+        f = nm.frame_from_pose(p, q, nm.DEFAULT_F, res, rmax, float(_logit(0.7)),
+                               float(_logit(0.4)), nm.ROUND_DOWN, w, h,
+                               nm.FILTER_VOXEL if filter_type == 1 else nm.FILTER_NONE)
+        t = oracle.Temp()
+        if filter_type == 1:
+            t.cast_rays(f, oracle.voxel_filter(f, cloud))
+        else:
+            t.cast_cloud(f, cloud)
+        omap.integrate(t, nm.ROUND_DOWN)
+    out = os.path.join(str(tmp_path), "map.bin")
+    print("host driver: 3 frames, filter", filter_type, flush=True)
+    r = subprocess.run([driver, "map", cfg, "cam", str(w), str(h), out] + args,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    go, gv, gm = _read_map(out)
+    oo, ov, omk = omap.export()
+    assert go.shape == oo.shape and len(go) > 0
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
+    assert np.array_equal(gm, omk)
