@@ -11,10 +11,12 @@
 // expected depth of the samples from k on, given the ray reaches k,
 //   D = sum_{i<k} w_i d_i + T_k E_k,  so  dD/dp_k = T_k (d_k - E_{k+1}),
 // and dp/dl = p (1 - p).  E is built by the backward recursion (positive
-// terms, no cancellation); T_k = exp(log T_N - sum_{j>=k} log(1 - p_j)),
-// from the total log-transmittance log T_N that the forward pass stores per
-// ray (one number per ray, no trace), with log(1 - p) = -softplus(l)
-// finite for every l.
+// terms, no cancellation); T_k = T_N / prod_{j>=k} (1 - p_j), from the
+// total log-transmittance log T_N that the forward pass stores per ray (one
+// number per ray, no trace).  The FP32 kernels carry transmittances as a
+// mantissa and a binary exponent (no underflow over any number of samples)
+// and take p and 1 - p from one exponential per sample; the FP64 kernels
+// sum log(1 - p) = -softplus(l) directly.
 #include "../../include/nanomap_amd_soft.h"
 #include "nm_internal.h"
 
@@ -127,6 +129,27 @@ __device__ __forceinline__ bool taps_inside(const SoftArgs<T>& A,
          tp.b[1] + 1 < A.ny && tp.b[2] >= 1 && tp.b[2] + 1 < A.nz;
 }
 
+// Three consecutive taps along z (k - 1 .. k + 1) of row (i, j).  Inside
+// the grid an FP32 row is one 12-byte load.
+typedef float float3_u __attribute__((ext_vector_type(3), aligned(4)));
+
+template <typename T>
+__device__ __forceinline__ void row3(const SoftArgs<T>& A, int i, int j,
+                                     int k, bool inside, T C[3]) {
+  if constexpr (sizeof(T) == 4) {
+    if (inside) {
+      const float3_u r = *reinterpret_cast<const float3_u*>(
+          A.grid + ((size_t)i * A.ny + j) * A.nz + (k - 1));
+      C[0] = r.x;
+      C[1] = r.y;
+      C[2] = r.z;
+      return;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) C[c] = coef(A, i, j, k + c - 1, inside);
+}
+
 // Value and gradient (per voxel) of the spline.
 template <typename T>
 __device__ __forceinline__ T spline(const SoftArgs<T>& A, const Taps<T>& tp,
@@ -138,10 +161,11 @@ __device__ __forceinline__ T spline(const SoftArgs<T>& A, const Taps<T>& tp,
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       T r = 0, rz = 0;
+      T C3[3];
+      row3(A, tp.b[0] + a - 1, tp.b[1] + b - 1, tp.b[2], in, C3);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T C = coef(A, tp.b[0] + a - 1, tp.b[1] + b - 1, tp.b[2] + c - 1,
-                         in);
+        const T C = C3[c];
         r += tp.w[2][c] * C;
         rz += tp.dw[2][c] * C;
       }
@@ -168,10 +192,10 @@ __device__ __forceinline__ T spline_value(const SoftArgs<T>& A,
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       T r = 0;
+      T C3[3];
+      row3(A, tp.b[0] + a - 1, tp.b[1] + b - 1, tp.b[2], in, C3);
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        r += tp.w[2][c] * coef(A, tp.b[0] + a - 1, tp.b[1] + b - 1,
-                               tp.b[2] + c - 1, in);
+      for (int c = 0; c < 3; ++c) r += tp.w[2][c] * C3[c];
       v += tp.w[0][a] * tp.w[1][b] * r;
     }
   }
@@ -219,6 +243,26 @@ __device__ __forceinline__ double log1m_sigmoid(double l) {
   return l > 0.0 ? -(l + log1p(exp(-l))) : -log1p(exp(l));
 }
 
+// FP32 occupancy of a sample from one exponential: p = sigmoid(l) and
+// q = 1 - p (each without cancellation), z = exp(-min(|l|, 80)).  The
+// clamp keeps q a normal float, so transmittances can be carried as a
+// mantissa and a binary exponent; it moves p and q by less than e^-80.
+__device__ __forceinline__ void occ32(float l, float& p, float& q, float& z,
+                                      bool& pos) {
+  pos = l >= 0.0f;
+  z = __expf(-fminf(fabsf(l), 80.0f));
+  const float inv = __builtin_amdgcn_rcpf(1.0f + z);
+  p = (pos ? 1.0f : z) * inv;
+  q = (pos ? z : 1.0f) * inv;
+}
+
+// m <- m * r renormalized to [0.5, 1), with the binary exponent in e.
+__device__ __forceinline__ void renorm_mul(float& m, int& e, float r) {
+  const float v = m * r;
+  e += __builtin_amdgcn_frexp_expf(v);
+  m = __builtin_amdgcn_frexp_mantf(v);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_soft_forward(SoftArgs<T> A,
                                                       T* __restrict__ depth,
@@ -228,21 +272,34 @@ __global__ __launch_bounds__(256) void k_soft_forward(SoftArgs<T> A,
   int pix;
   if (!ray_setup(A, c, blockIdx.x, threadIdx.x, e, dv, dir, &pix)) return;
   T tr = 1, D = 0;
-  double lt = 0.0;
+  double lt = 0.0;            // FP64: sum of log(1 - p)
+  float tm = 1.0f;            // FP32: transmittance as tm * 2^te
+  int te = 0;
   for (int i = 0; i < A.ns; ++i) {
     const T d = (T(i) + T(0.5)) * A.step;
     const T q[3] = {e[0] + dv[0] * d, e[1] + dv[1] * d, e[2] + dv[2] * d};
     Taps<T> tp;
     taps_at(q, tp);
     const T l = spline_value(A, tp);
-    const T p = sigmoid_(l);
-    const T w = p * tr;
-    D += w * d;
-    tr *= T(1) - p;
-    if (logt) lt += (double)log1m_sigmoid(l);
+    if constexpr (sizeof(T) == 4) {
+      float p, om, z;
+      bool pos;
+      occ32(l, p, om, z, pos);
+      D += p * tr * d;
+      tr *= om;
+      renorm_mul(tm, te, om);
+    } else {
+      const T p = sigmoid_(l);
+      const T w = p * tr;
+      D += w * d;
+      tr *= T(1) - p;
+      if (logt) lt += (double)log1m_sigmoid(l);
+    }
   }
   const size_t ray = (size_t)c * A.width * A.height + pix;
   depth[ray] = D;
+  if constexpr (sizeof(T) == 4)
+    lt = (double)logf(tm) + (double)te * 0.69314718055994531;
   if (logt) logt[ray] = lt;
 }
 
@@ -269,6 +326,11 @@ __global__ __launch_bounds__(256) void k_soft_backward(
   if (valid) {
     const size_t ray = (size_t)c * A.width * A.height + pix;
     double lt = logt[ray];  // log T_N, then log T_k on the way back
+    // FP32: T_k as tm * 2^te, from T_N up by T_k = T_{k+1} / (1 - p_k)
+    const double y = lt * 1.4426950408889634;  // log2 T_N
+    const double fy = floor(y);
+    float tm = exp2f((float)(y - fy));
+    int te = (int)fy;
     gout = gdepth ? gdepth[ray] : T(0);
     const T inv_res = T(A.inv_res);
     T E = 0;  // E_{k+1}
@@ -279,10 +341,20 @@ __global__ __launch_bounds__(256) void k_soft_backward(
       taps_at(q, tp);
       T g[3];
       const T l = spline(A, tp, g);
-      const T p = sigmoid_(l);
-      const T om = T(1) - p;
-      lt -= (double)log1m_sigmoid(l);
-      const T tk = exp_t((T)lt);
+      T p, om, tk;
+      if constexpr (sizeof(T) == 4) {
+        float z;
+        bool pos;
+        occ32(l, p, om, z, pos);
+        // 1 / (1 - p) = 1 + e^l
+        renorm_mul(tm, te, pos ? 1.0f + __builtin_amdgcn_rcpf(z) : 1.0f + z);
+        tk = __builtin_amdgcn_ldexpf(tm, te);
+      } else {
+        p = sigmoid_(l);
+        om = T(1) - p;
+        lt -= (double)log1m_sigmoid(l);
+        tk = exp_t((T)lt);
+      }
       const T dDdl = tk * (d - E) * p * om;
       E = p * d + om * E;
       if (dDdl == T(0)) continue;
