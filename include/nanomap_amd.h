@@ -75,8 +75,8 @@ typedef struct nm_frame {
   double eye[3];        /* sensor origin in index space (Handler.cpp:34)    */
   float grid_res;       /* voxel size, metres                               */
   float max_time;       /* max range / grid_res (Handler.cpp:39)            */
-  float logodds_hit;    /* added on the end voxel of a ray                  */
-  float logodds_miss;   /* added on every voxel a ray passes                */
+  float logodds_hit;    /* added on the end voxel of a ray (|.| <= 1e27)    */
+  float logodds_miss;   /* added on every voxel a ray passes (|.| <= 1e27)  */
   int rounding;         /* NM_ROUND_*                                       */
   int filter;           /* NM_FILTER_*                                      */
   int width, height;    /* cloud layout (pcl_width, pcl_height)             */
@@ -104,8 +104,9 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out);
 int nm_ctx_destroy(nm_ctx* ctx);
 
 /* ---- map: map::Map (include/nanomap/map/OccupancyMap.h:35-61) ---------- */
-/* clamp_min/clamp_max: emptyClampingThreshold/occupiedClampingThreshold;
- * hit_thr/miss_thr: logOddsHitThreshold/logOddsMissThreshold. */
+/* clamp_min/clamp_max: emptyClampingThreshold/occupiedClampingThreshold
+ * (finite, |.| <= 1e27); hit_thr/miss_thr: logOddsHitThreshold/
+ * logOddsMissThreshold. */
 int nm_map_create(int device, float grid_res, float clamp_min,
                   float clamp_max, float hit_thr, float miss_thr,
                   int leaf_capacity, nm_map** out);

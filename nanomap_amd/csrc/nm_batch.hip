@@ -140,6 +140,8 @@ int check_frames(nm_batch* b, const nm_agent_frame* f, int n) {
       return bfail(NM_EINVAL, "nm_batch: frames need one valid rounding mode");
     if (fr.filter != NM_FILTER_NONE)
       return bfail(NM_EINVAL, "nm_batch: frames must use NM_FILTER_NONE");
+    if (!nm::logodds_ok(fr.logodds_hit) || !nm::logodds_ok(fr.logodds_miss))
+      return bfail(NM_EINVAL, "nm_batch: log-odds must be finite, magnitude <= 1e27");
     for (int a = 0; a < 3; ++a)
       if (!std::isfinite(fr.eye[a]) || fabs(fr.eye[a]) > 1e8)
         return bfail(NM_EINVAL, "nm_batch: eye must be finite");

@@ -232,7 +232,8 @@ int nm_map_create(int device, float grid_res, float clamp_min,
                   int leaf_capacity, nm_map** out) {
   if (!out) return fail(NM_EINVAL, "nm_map_create: null out");
   *out = nullptr;
-  if (!(grid_res > 0) || leaf_capacity <= 0 || !(clamp_min <= clamp_max))
+  if (!(grid_res > 0) || leaf_capacity <= 0 || !(clamp_min <= clamp_max) ||
+      !nm::logodds_ok(clamp_min) || !nm::logodds_ok(clamp_max))
     return fail(NM_EINVAL, "nm_map_create: bad argument");
   NM_HIP(hipSetDevice(device));
   nm_map* m = new nm_map();
@@ -373,6 +374,8 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   if (!(frame->grid_res > 0) || !(frame->max_time > 0) ||
       frame->max_time > ctx->max_time)
     return fail(NM_EINVAL, "nm_cast: max_time must be in (0, context max_time]");
+  if (!nm::logodds_ok(frame->logodds_hit) || !nm::logodds_ok(frame->logodds_miss))
+    return fail(NM_EINVAL, "nm_cast: log-odds must be finite, magnitude <= 1e27");
   if (frame->rounding != NM_ROUND_NEAREST && frame->rounding != NM_ROUND_DOWN)
     return fail(NM_EINVAL, "nm_cast: bad rounding mode");
   if (frame->filter != NM_FILTER_NONE && frame->filter != NM_FILTER_VOXEL)

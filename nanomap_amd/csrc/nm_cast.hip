@@ -724,7 +724,7 @@ __device__ bool map_insert(const MapView& m, int x, int y, int z, int idx) {
 
 template <int M>
 __device__ __noinline__ float fold_call(float v, float a, uint32_t k) {
-  return fold<M>(v, a, k);
+  return fold<M, true>(v, a, k);
 }
 
 // fold<M> with its common short case (a few steps) inline and the rest out
@@ -732,8 +732,8 @@ __device__ __noinline__ float fold_call(float v, float a, uint32_t k) {
 template <int M>
 __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
   if (k <= 2) {
-    if (k > 0) v = add<M>(v, a);
-    if (k > 1) v = add<M>(v, a);
+    if (k > 0) v = add<M, true>(v, a);
+    if (k > 1) v = add<M, true>(v, a);
     return v;
   }
   return fold_call<M>(v, a, k);
@@ -760,14 +760,14 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     while (nzm) {
       const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
       nzm &= nzm - 1;
-      acc = fold<M>(acc, hit, p - pos);
+      acc = fold<M, true>(acc, hit, p - pos);
       const uint32_t gp = (uint32_t)__shfl((int)cur, (int)p, 64);
-      acc = fold<M>(acc, miss, gp);
-      acc = add<M>(acc, hit);
+      acc = fold<M, true>(acc, miss, gp);
+      acc = add<M, true>(acc, hit);
       used += gp;
       pos = p + 1;
     }
-    acc = fold<M>(acc, hit, n - pos);
+    acc = fold<M, true>(acc, hit, n - pos);
     cur = nxt;
   }
   *used_out = used;
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(256) void k_temp(
     if (!touched || heavy) continue;
     float tv;
     if (j == 0) {
-      tv = fold<M>(0.0f, miss, k);
+      tv = fold<M, true>(0.0f, miss, k);
     } else {
       // gaps are loaded 4 at a time so the loads overlap
       uint32_t used = 0;
@@ -837,11 +837,12 @@ __global__ __launch_bounds__(256) void k_temp(
               acc = fold_small<M>(acc, miss, gm[u]);
               used += gm[u];
             }
-            acc = add<M>(acc, hit);
+            acc = add<M, true>(acc, hit);
           }
         }
       }
-      tv = fold<M>(acc, miss, k - used);
+      // (used <= k: every gap counts a miss that k counts too)
+      tv = fold<M, true>(acc, miss, k >= used ? k - used : 0u);
     }
     tiles[2 * cell] = bits_of(tv);
     tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
@@ -869,7 +870,7 @@ __global__ __launch_bounds__(256) void k_heavy(
     uint32_t used = 0;
     const float acc =
         wave_hit_sequence<M>(gap + s0, j, J.miss, J.hit, &used);
-    const float tv = fold<M>(acc, J.miss, k - used);
+    const float tv = fold<M, true>(acc, J.miss, k >= used ? k - used : 0u);
     if (lane == 0) {
       tiles[2 * cell] = bits_of(tv);
       tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
@@ -938,7 +939,7 @@ __global__ __launch_bounds__(512) void k_apply(
       uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
       bool act = (old_w >> (li & 31)) & 1u;
       if (nz) {
-        float v = add<M>(old_v, tv);
+        float v = add<M, true>(old_v, tv);
         if (v > m.clamp_max) {
           v = m.clamp_max;
         } else if (v < m.clamp_min) {

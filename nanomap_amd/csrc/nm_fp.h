@@ -18,6 +18,7 @@
 #ifndef NM_FP_H
 #define NM_FP_H
 
+#include <limits>
 #include <stdint.h>
 #include <string.h>
 
@@ -52,6 +53,11 @@ NM_HD float float_of(uint32_t u) {
   float x;
   memcpy(&x, &u, 4);
   return x;
+}
+
+template <class T>
+NM_HD T abs_t(T x) {
+  return x < T(0) ? -x : x;
 }
 
 // Next representable value toward -inf, for finite nonzero x.
@@ -93,10 +99,47 @@ NM_HD float sqrt_(float x) {
 #endif
 }
 
+// Largest finite value of T.
+template <class T>
+NM_HD T max_finite();
+template <>
+NM_HD float max_finite<float>() {
+  return 3.40282347e38f;
+}
+template <>
+NM_HD double max_finite<double>() {
+  return 1.7976931348623157e308;
+}
+
+// Smallest-magnitude negative value of T (the round-down of a negative
+// result below it in magnitude).
+template <class T>
+NM_HD T neg_tiny() {
+  return -std::numeric_limits<T>::denorm_min();
+}
+
 // Sum rounded toward -inf.  An exact zero sum of nonzero operands is -0
-// in that mode; +0 only when both operands are +0.
+// in that mode; +0 only when both operands are +0.  A finite sum above the
+// largest finite value rounds down to it.
 template <class T>
 NM_HD T add_down(T a, T b) {
+  T s = a + b;
+  if (s == T(0)) {
+    const bool both_pos_zero = (a == T(0) && b == T(0) &&
+                                !(bits_of(a) >> (sizeof(T) * 8 - 1)) &&
+                                !(bits_of(b) >> (sizeof(T) * 8 - 1)));
+    return both_pos_zero ? T(0) : -T(0);
+  }
+  if (s > max_finite<T>() && a <= max_finite<T>() && b <= max_finite<T>())
+    return max_finite<T>();  // +inf from finite operands
+  const T bb = s - a;
+  const T err = (a - (s - bb)) + (b - bb);
+  return err < T(0) ? next_down(s) : s;
+}
+
+// add_down for operands whose sum is known to be finite (no overflow case).
+template <class T>
+NM_HD T add_down_finite(T a, T b) {
   T s = a + b;
   if (s == T(0)) {
     const bool both_pos_zero = (a == T(0) && b == T(0) &&
@@ -120,10 +163,17 @@ NM_HD double add_down_pos(double a, double b) {
   return err < 0.0 ? double_of(bits_of(s) - 1) : s;
 }
 
+// (An overflowing product or quotient comes out right through next_down:
+// the residual against infinity has the sign that steps +inf down to the
+// largest finite value and keeps -inf.)
 template <class T>
 NM_HD T mul_down(T a, T b) {
   const T p = a * b;
-  if (p == T(0)) return p;
+  if (p == T(0)) {
+    // a nonzero negative product below the smallest subnormal
+    const bool neg = (a < T(0)) != (b < T(0));
+    return (neg && a != T(0) && b != T(0)) ? neg_tiny<T>() : p;
+  }
   const T err = fma_(a, b, -p);
   return err < T(0) ? next_down(p) : p;
 }
@@ -131,7 +181,11 @@ NM_HD T mul_down(T a, T b) {
 template <class T>
 NM_HD T div_down(T a, T b) {
   const T q = a / b;
-  if (q == T(0) || b == T(0)) return q;
+  if (b == T(0)) return q;
+  if (q == T(0)) {
+    const bool neg = (a < T(0)) != (b < T(0));
+    return (neg && a != T(0) && !(abs_t(b) > max_finite<T>())) ? neg_tiny<T>() : q;
+  }
   const T r = fma_(-q, b, a);  // exact: a - q*b
   const bool q_above = (b > T(0)) ? (r < T(0)) : (r > T(0));
   return q_above ? next_down(q) : q;
@@ -146,9 +200,12 @@ NM_HD T sqrt_down(T x) {
 }
 
 // Mode-dispatched operations; M is a compile-time constant in kernels.
-template <int M, class T>
+// kFinite: the caller guarantees a finite sum (bounded operands), which
+// drops the overflow case from the round-down sum.
+template <int M, bool kFinite = false, class T>
 NM_HD T add(T a, T b) {
-  return M == kDown ? add_down(a, b) : a + b;
+  if (M != kDown) return a + b;
+  return kFinite ? add_down_finite(a, b) : add_down(a, b);
 }
 template <int M, class T>
 NM_HD T sub(T a, T b) {
@@ -193,20 +250,28 @@ NM_HD double rint_(double x) {
 }
 NM_HD double abs_(double x) { return x < 0 ? -x : x; }
 
-template <int M>
+//
+// kFinite: v, a and every partial sum are known to be finite (bounded
+// inputs, as in the cast kernels); otherwise an infinite or NaN v returns
+// at once, since it stays what it is under any further step.
+template <int M, bool kFinite = false>
 NM_HD float fold(float v, float a, uint32_t k) {
-  if (a == 0.0f) return k ? add<M>(v, a) : v;
+  if (a == 0.0f) return k ? add<M, kFinite>(v, a) : v;
+  if (!kFinite && !(abs_((double)v) <= 3.4028234663852886e38))
+    return k ? add<M>(v, a) : v;
   // short runs: plain rounded steps are cheaper than the run analysis
   while (k > 0 && k <= 24) {
-    v = add<M>(v, a);
+    v = add<M, kFinite>(v, a);
     --k;
   }
   while (k > 0) {
+    if (!kFinite && !(abs_((double)v) <= 3.4028234663852886e38))
+      return add<M>(v, a);
     const double av = abs_((double)v), aa = abs_((double)a);
     // Single steps while |v| is small relative to the increment, so every
     // jump below stays clear of zero crossings and subnormals.
     if (av < 2.0 * aa || av < 1e-30) {
-      v = add<M>(v, a);
+      v = add<M, kFinite>(v, a);
       --k;
       continue;
     }
@@ -220,7 +285,7 @@ NM_HD float fold(float v, float a, uint32_t k) {
       dq = floor_(q);
     } else {
       if (q - floor_(q) == 0.5) {  // tie: step depends on parity of v
-        v = add<M>(v, a);
+        v = add<M, kFinite>(v, a);
         --k;
         continue;
       }
@@ -255,7 +320,7 @@ NM_HD float fold(float v, float a, uint32_t k) {
       }
     }
     if (n <= 0) {
-      v = add<M>(v, a);
+      v = add<M, kFinite>(v, a);
       --k;
       continue;
     }

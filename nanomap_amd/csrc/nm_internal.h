@@ -267,6 +267,12 @@ inline int rec_words_for_cube(const Cube& c) {
   return rec_words_for((float)(8 * (r - 1) - 2));
 }
 
+// Log-odds increments and clamps are bounded so that every temp sum (at
+// most 2^32 terms) and every map value stays a finite float, which the
+// round-down sums of the update rely on.
+constexpr float kMaxLogOdds = 1e27f;
+inline bool logodds_ok(float v) { return std::fabs(v) <= kMaxLogOdds; }
+
 // Bits needed for keys below max_key.
 inline int sort_bits_for(int64_t max_key) {
   int bits = 1;

@@ -55,6 +55,24 @@ def test_fold_host_matches_oracle_sequential_sums(nm, oracle_lib):
             assert g == o, (mode, v, a, k)
 
 
+def test_fold_host_overflow_and_non_finite(nm, oracle_lib):
+    big = float(np.finfo(np.float32).max)
+    # sums that overflow: round-down keeps FLT_MAX going up, reaches -inf
+    # going down; an infinite or NaN start stays put; every case returns
+    # at once even for 2^32 - 1 steps
+    cases = [(big, 1e38, 5), (-big, -1e38, 5), (3e38, 2e37, 40),
+             (float("inf"), -0.4, 7), (float("-inf"), 0.85, 7)]
+    for mode in (0, 1):
+        for v, a, k in cases:
+            g = np.float32(nm.fold(v, a, k, mode)).view(np.uint32)
+            o = np.float32(oracle.fold(v, a, k, mode)).view(np.uint32)
+            assert g == o, (mode, v, a, k)
+        for v in (float("inf"), float("-inf")):
+            assert nm.fold(v, -0.4, 2 ** 32 - 1, mode) == v
+        assert np.isnan(nm.fold(float("nan"), 0.85, 2 ** 32 - 1, mode))
+        assert nm.fold(-big, -1e38, 2 ** 32 - 1, mode) == float("-inf")
+
+
 def test_frame_from_pose_closed_forms(nm):
     # identity quaternion, identity frame transform -> R = I, eye = p / res
     I = np.eye(3, dtype=np.float32)

@@ -256,6 +256,13 @@ def test_bad_arguments_raise(gpu):
     big.max_time = 60.0
     with pytest.raises(nm.NanoMapError):
         ctx.cast(big, np.zeros((10, 3), np.float32))  # beyond context range
+    for lh, lm in ((float("inf"), -0.4), (0.85, float("nan")), (2e27, -0.4)):
+        bad = f.copy()
+        bad.logodds_hit, bad.logodds_miss = lh, lm
+        with pytest.raises(nm.NanoMapError):
+            ctx.cast(bad, np.zeros((10, 3), np.float32))  # unbounded log-odds
+    with pytest.raises(nm.NanoMapError):
+        nm.Map(0.1, -2.0, float("inf"))
     with pytest.raises(nm.NanoMapError) as e:
         ctx.update(g)  # no pending cast
     assert e.value.code == 6
