@@ -183,7 +183,7 @@ int build_layout(nm_batch* b, const nm_agent_frame* f, int n) {
   b->layout_valid = false;
   std::vector<nm::Job> J(n);
   int64_t cube_off = 0, ray_off = 0;
-  int64_t item_off = 0, rec_off = 0;
+  int64_t item_off = 0;
   for (int i = 0; i < n; ++i) {
     nm::Job& j = J[i];
     j = nm::Job{};
@@ -206,19 +206,14 @@ int build_layout(nm_batch* b, const nm_agent_frame* f, int n) {
     j.item_off = (int)item_off;
     j.nitems = nm::job_items(j, j.layout);
     item_off += j.nitems;
-    j.rec_words = nm::rec_words_for_cube(cube);
-    j.rec_off = rec_off;
-    rec_off += (int64_t)j.nitems * 256 * j.rec_words;
   }
   if (item_off > (1 << 30))
     return bfail(NM_EINVAL, "nm_batch: too many work items");
   // grow the workspace if this shape needs more cube entries or items
-  if (cube_off > b->dims.cube_cap || item_off > b->dims.item_cap ||
-      rec_off > b->dims.rec_cap) {
+  if (cube_off > b->dims.cube_cap || item_off > b->dims.item_cap) {
     nm::WorkspaceDims d = b->dims;
     d.cube_cap = std::max<int64_t>(d.cube_cap, cube_off);
     d.item_cap = std::max<int>(d.item_cap, (int)(item_off + item_off / 4));
-    d.rec_cap = std::max<int64_t>(d.rec_cap, rec_off + rec_off / 4);
     BH(hipDeviceSynchronize());
     nm::workspace_free(&b->ws);
     hipError_t e = nm::workspace_alloc(d, &b->ws);

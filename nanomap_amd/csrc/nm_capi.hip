@@ -33,7 +33,6 @@ struct nm_ctx {
   int cube_side_max = 0;
   int64_t cube_cap = 0;
   int tile_cap = 0;
-  int64_t rec_cap = 0;  // step-record words allocated
   int cu_count = 256;
   size_t staging_bytes = 0;
   nm::Workspace b{};
@@ -194,9 +193,6 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   d.ray_cap = (int64_t)P;
   // 16x16 items: ceil(W/16)*ceil(H/16) <= P/256 + (W+H)/16 + 1, W+H <= P+1
   d.item_cap = (int)(P / 16 + P / 256 + 64);
-  // step records for images whose tiles are nearly full (any 16x16-tiled
-  // image with (W + H) / 16 < 64); nm_cast grows them for other shapes
-  d.rec_cap = (int64_t)(P / 256 + 64) * 256 * nm::rec_words_for(cfg->max_time);
   d.max_jobs = 1;
   d.host_clouds = true;
   c->staging_bytes = P * 16;
@@ -204,7 +200,6 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess &&
       prop.multiProcessorCount > 0)
     c->cu_count = prop.multiProcessorCount;
-  c->rec_cap = d.rec_cap;
   hipError_t e = nm::workspace_alloc(d, &c->b);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&c->host_counters, sizeof(nm::FrameCounters));
@@ -432,17 +427,6 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.ray_off = 0;
   p.item_off = 0;
   p.nitems = nm::job_items(p, p.layout);
-  p.rec_words = nm::rec_words_for(frame->max_time);
-  p.rec_off = 0;
-  const int64_t rec_need = (int64_t)p.nitems * 256 * p.rec_words;
-  if (rec_need > ctx->rec_cap) {
-    NM_HIP(hipStreamSynchronize(s));
-    NM_HIP(hipFree(ctx->b.steps));
-    ctx->b.steps = nullptr;
-    ctx->rec_cap = 0;
-    NM_HIP(hipMalloc((void**)&ctx->b.steps, (size_t)rec_need * 4));
-    ctx->rec_cap = rec_need;
-  }
 
   nm::Pipeline& pl = ctx->pipe;
   pl = nm::Pipeline{};

@@ -12,7 +12,7 @@
 //  1 discover   one lane per point: ray set-up + voxel DDA (nm_dda.h);
 //               marks every 8^3 leaf the ray enters in the job's dense
 //               cube table around the eye, records the cell of the ray's
-//               hit voxel and the axis of every step (2 bits each).
+//               hit voxel.
 //  2 compact    prefix sum over all jobs' cube tables -> dense tile slots,
 //               in ascending (job, cube) order (deterministic).
 //  3 hit sort   (tile cell, point) pairs of all hits, stable radix sort:
@@ -20,8 +20,7 @@
 //  4 segments   per hit voxel: end of its run of sorted hits into its
 //               tile (the start into hit_start), hit bit in the
 //               tile's 512-bit mask, zeroed gap counters.
-//  5 accumulate one lane per point again, replaying the recorded steps
-//               (integer adds, no DDA): integer miss counts into the
+//  5 accumulate one lane per point again: integer miss counts into the
 //               tiles (LDS-aggregated per work item).  A miss in a voxel
 //               that also holds hits increments the gap counter of the
 //               first later hit (binary search in the sorted hits), so the
@@ -181,21 +180,10 @@ __device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
                        s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
 }
 
-// Step record of the ray of thread `tid` in item `item` of job J: word k
-// at the returned pointer + k * 64 (the words of a wavefront's rays
-// interleave, so a wavefront reads or writes 64 consecutive words).
-__device__ __forceinline__ size_t rec_base(const Job& J, int item, int tid) {
-  return (size_t)J.rec_off +
-         ((size_t)(item - J.item_off) * kCastThreads + (size_t)(tid & ~63)) *
-             (size_t)J.rec_words +
-         (size_t)(tid & 63);
-}
-
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
     const Job* __restrict__ jobs, ItemQueue q, int32_t* __restrict__ leaf_slot,
-    uint32_t* __restrict__ ray_hit, uint32_t* __restrict__ steps,
-    uint32_t* __restrict__ steps_hdr, uint64_t* __restrict__ part) {
+    uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part) {
   int round = 0;
   for (;;) {
     const int item = take_item(q, round);
@@ -204,7 +192,7 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
     int32_t* __restrict__ ls = leaf_slot + J.cube_off;
     const int i = point_of(J, item - J.item_off, threadIdx.x);
     uint32_t hit_cell = kInvalid;
-    uint32_t visits = 0, hdr = 0;
+    uint32_t visits = 0;
     int valid = 0, errors = 0;
     RayDir r;
     if (i >= 0 && ray_of<M>(J, i, r)) {
@@ -216,22 +204,9 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
         ++errors;
       else
         ls[c] = -2;
-      // every step (a miss on the voxel it leaves) is recorded as the
-      // axis it moves along, so the accumulate pass replays the ray
-      // without the DDA
-      uint32_t* __restrict__ rec = steps + rec_base(J, item, threadIdx.x);
-      const uint32_t cap = (uint32_t)J.rec_words * 16u;
-      uint32_t acc = 0;
       int px = s.vx, py = s.vy, pz = s.vz;
       while (dda_step<M>(s, r.L)) {
-        // miss on (px, py, pz)
-        const uint32_t code = s.vx != px ? 0u : (s.vy != py ? 1u : 2u);
-        acc |= code << (2u * (visits & 15u));
-        ++visits;
-        if ((visits & 15u) == 0) {
-          if (visits <= cap) rec[(size_t)((visits >> 4) - 1) * 64] = acc;
-          acc = 0;
-        }
+        ++visits;  // miss on (px, py, pz)
         if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7) {
           c = cube_index(J.cube, s.vx, s.vy, s.vz);
           if (c < 0)
@@ -243,10 +218,6 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
         py = s.vy;
         pz = s.vz;
       }
-      if ((visits & 15u) && visits <= cap) rec[(size_t)(visits >> 4) * 64] = acc;
-      if (visits > cap) ++errors;  // cannot happen: rec_words_for bounds it
-      hdr = min(visits, cap) | (uint32_t)(s.sx + 1) << 24 |
-            (uint32_t)(s.sy + 1) << 26 | (uint32_t)(s.sz + 1) << 28;
       if (s.t < (double)J.max_time) {
         ++visits;
         const int ce = cube_index(J.cube, px, py, pz);
@@ -254,7 +225,6 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
           hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(px, py, pz);
       }
     }
-    steps_hdr[(size_t)item * kCastThreads + threadIdx.x] = hdr;
     if (i >= 0) ray_hit[(size_t)J.ray_off + i] = hit_cell;
     const uint64_t v[4] = {visits, (uint64_t)valid,
                            (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
@@ -490,15 +460,13 @@ __device__ __forceinline__ int slot_of(const Cube& cube,
   return c >= 0 ? leaf_slot[c] : -1;
 }
 
-template <int S>
+template <int M, int S>
 __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const Job* __restrict__ jobs, ItemQueue q,
     const int32_t* __restrict__ leaf_slot, uint32_t* __restrict__ tiles,
     const uint32_t* __restrict__ hit_mask,
     const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
-    const uint32_t* __restrict__ steps,
-    const uint32_t* __restrict__ steps_hdr,
     FrameCounters* __restrict__ cnt) {
   constexpr int kSlots = 1 << S;
   __shared__ uint32_t s_key[kSlots];
@@ -517,27 +485,13 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const int32_t* __restrict__ ls = leaf_slot + J.cube_off;
     const int i = point_of(J, item - J.item_off, threadIdx.x);
     const uint32_t gi = (uint32_t)(J.ray_off + i);
-    const uint32_t hdr = steps_hdr[(size_t)item * kCastThreads + threadIdx.x];
-    const int n = (int)(hdr & 0xffffffu);
-    if (n > 0) {
-      // replay the discover pass's steps from the eye voxel (every ray of
-      // a job starts at floor(eye))
-      const int sx = (int)((hdr >> 24) & 3u) - 1;
-      const int sy = (int)((hdr >> 26) & 3u) - 1;
-      const int sz = (int)((hdr >> 28) & 3u) - 1;
-      const uint32_t* __restrict__ rec = steps + rec_base(J, item, threadIdx.x);
-      uint32_t word = rec[0];
-      uint32_t next = n > 16 ? rec[64] : 0u;
-      int px = (int)floor_(J.eye[0]), py = (int)floor_(J.eye[1]),
-          pz = (int)floor_(J.eye[2]);
+    RayDir r;
+    if (i >= 0 && ray_of<M>(J, i, r)) {
+      Dda s;
+      dda_init<M>(J.eye, r, s);
+      int px = s.vx, py = s.vy, pz = s.vz;
       int entry = slot_of(J.cube, ls, px, py, pz);
-      for (int k = 0; k < n; ++k) {
-        if (k && (k & 15) == 0) {
-          word = next;
-          if (k + 16 < n) next = rec[(size_t)((k >> 4) + 1) * 64];
-        }
-        const uint32_t code = word & 3u;
-        word >>= 2;
+      while (dda_step<M>(s, r.L)) {
         // miss on (px, py, pz)
         if (entry < 0) {
           ++errors;
@@ -553,7 +507,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
           if (hit_voxel) {
             // a voxel holding hits: count the miss directly and mark the
             // gap before the first later hit of this voxel (its hits are
-            // sorted[s0, s0 + j) in ascending point order)
+            // sorted[s0, end) in ascending point order)
             const uint32_t s0 = hit_start[cell];
             const uint32_t end = tiles[2 * (size_t)cell + 1];
             uint32_t lo = s0, hi = end;
@@ -570,14 +524,11 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
             wave_lds_count<S>(s_key, s_cnt, tiles, cell);
           }
         }
-        const int nx = px + (code == 0u ? sx : 0);
-        const int ny = py + (code == 1u ? sy : 0);
-        const int nz = pz + (code == 2u ? sz : 0);
-        if (((nx ^ px) | (ny ^ py) | (nz ^ pz)) & ~7)
-          entry = slot_of(J.cube, ls, nx, ny, nz);
-        px = nx;
-        py = ny;
-        pz = nz;
+        if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
+          entry = slot_of(J.cube, ls, s.vx, s.vy, s.vz);
+        px = s.vx;
+        py = s.vy;
+        pz = s.vz;
       }
     }
     __syncthreads();
@@ -1102,13 +1053,11 @@ hipError_t launch_discover(const Pipeline& p, const Workspace& b,
   int grid = 0;
   const ItemQueue q = cast_queue(p, b, 0, &grid);
   if (p.rounding == kDown)
-    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.ray_hit, b.steps, b.steps_hdr,
-        b.part_discover);
+    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(b.jobs, q, b.leaf_slot,
+                                                    b.ray_hit, b.part_discover);
   else
     k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.ray_hit, b.steps, b.steps_hdr,
-        b.part_discover);
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover);
   return hipGetLastError();
 }
 
@@ -1149,12 +1098,12 @@ hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
   return hipGetLastError();
 }
 
-template <int S>
+template <int M, int S>
 void accumulate_as(const Pipeline& p, const Workspace& b, int sorted_buf,
                    const ItemQueue& q, int grid, hipStream_t s) {
-  k_accumulate<S><<<grid, kCastThreads, 0, s>>>(
+  k_accumulate<M, S><<<grid, kCastThreads, 0, s>>>(
       b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
-      b.vals[sorted_buf], b.gap, b.steps, b.steps_hdr, b.counters);
+      b.vals[sorted_buf], b.gap, b.counters);
 }
 
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
@@ -1162,10 +1111,18 @@ hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
   if (p.nitems <= 0) return hipSuccess;
   int grid = 0;
   const ItemQueue q = cast_queue(p, b, 1, &grid);
-  if (p.tuning.lds_slots == 2048)
-    accumulate_as<11>(p, b, sorted_buf, q, grid, s);
-  else
-    accumulate_as<12>(p, b, sorted_buf, q, grid, s);
+  const bool small = p.tuning.lds_slots == 2048;
+  if (p.rounding == kDown) {
+    if (small)
+      accumulate_as<kDown, 11>(p, b, sorted_buf, q, grid, s);
+    else
+      accumulate_as<kDown, 12>(p, b, sorted_buf, q, grid, s);
+  } else {
+    if (small)
+      accumulate_as<kNearest, 11>(p, b, sorted_buf, q, grid, s);
+    else
+      accumulate_as<kNearest, 12>(p, b, sorted_buf, q, grid, s);
+  }
   return hipGetLastError();
 }
 

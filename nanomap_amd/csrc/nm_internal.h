@@ -104,8 +104,7 @@ struct Job {
   int ray_off;       // global index of the job's first point
   int item_off;      // first cast work item of the job
   int nitems;        // cast work items of the job
-  int rec_words;     // step-record words per ray (rec_words_for(max_time))
-  int64_t rec_off;   // first step-record word of the job's first item
+  int pad_;
 };
 using CastParams = Job;
 
@@ -134,15 +133,6 @@ struct Workspace {
   uint32_t* hit_mask;   // [tile_cap * 16]  voxels holding a hit
   uint32_t* hit_start;  // [tile_cap * 512] first sorted hit of a hit voxel
   uint32_t* ray_hit;    // [ray_cap]  job-cube cell of each ray's hit
-  uint32_t* steps;      // [rec_cap]  step records: the axis (2 bits) of
-                        //             every DDA step of every ray, 16 per
-                        //             word; word k of the ray of thread
-                        //             (item, tid) at rec_off + (item *
-                        //             256 + (tid & ~63)) * rec_words +
-                        //             k * 64 + (tid & 63)
-  uint32_t* steps_hdr;  // [item_cap * 256]  per item thread: step count
-                        //             (bits 0-23), step sign + 1 per axis
-                        //             (2 bits each from bit 24)
   uint32_t* keys[2];    // [ray_cap]  radix sort buffers
   uint32_t* vals[2];
   uint32_t* gap;        // [ray_cap]  misses before each sorted hit
@@ -170,7 +160,6 @@ struct WorkspaceDims {
   int tile_cap = 0;      // tile slots
   int64_t ray_cap = 0;   // points of all jobs
   int item_cap = 0;      // cast work items of all jobs
-  int64_t rec_cap = 0;   // step-record words of all jobs
   int max_jobs = 1;
   bool host_clouds = false;  // staging for host clouds + voxel filter
 };
@@ -249,23 +238,6 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
                                const Workspace& b, hipStream_t s);
 
 void set_error(const std::string& msg);
-
-// Step-record words per ray for rays shorter than max_time voxels.  Axis
-// a is crossed at most L |d_a| (1 + 1e-11) + 1 times over [0, L] (its
-// crossing times grow by 1/|d_a| each, less at most one rounding of a
-// value below 2^12), and |d|_1 <= sqrt(3) |d|_2, so a ray takes at most
-// max_time * sqrt(3) * (1 + 1e-10) + 3 steps.
-inline int rec_words_for(float max_time) {
-  const double steps = ceil((double)max_time * 1.7320509) + 4.0;
-  return (int)((steps + 15.0) / 16.0);
-}
-
-// Step-record words per ray for any frame whose leaf cube is c (a batch
-// layout is kept while the cube sizes stay the same).
-inline int rec_words_for_cube(const Cube& c) {
-  const int r = (c.side - 1) / 2;  // frame_cube: ceil((max_time + 2) / 8) + 1
-  return rec_words_for((float)(8 * (r - 1) - 2));
-}
 
 // Log-odds increments and clamps are bounded so that every temp sum (at
 // most 2^32 terms) and every map value stays a finite float, which the

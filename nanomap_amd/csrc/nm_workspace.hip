@@ -35,8 +35,6 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.hit_mask, T * kMaskWords));
   A(dalloc(&b.hit_start, T * 512));
   A(dalloc(&b.ray_hit, P));
-  A(dalloc(&b.steps, (size_t)(d.rec_cap > 0 ? d.rec_cap : 1)));
-  A(dalloc(&b.steps_hdr, I * 256));
   A(dalloc(&b.keys[0], P));
   A(dalloc(&b.keys[1], P));
   A(dalloc(&b.vals[0], P));
@@ -79,8 +77,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
 void workspace_free(Workspace* b) {
   void* ptrs[] = {b->leaf_slot,  b->slot_leaf,   b->slot_job,   b->map_idx,
                   b->map_new,    b->slot_stats,  b->tiles,      b->hit_mask,
-                  b->hit_start,  b->ray_hit,     b->steps,      b->steps_hdr,
-                  b->keys[0],    b->keys[1],
+                  b->hit_start,  b->ray_hit,     b->keys[0],    b->keys[1],
                   b->vals[0],    b->vals[1],     b->gap,        b->heavy,
                   b->radix_hist,
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
