@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
       else
         ls[c] = -2;
       int px = s.vx, py = s.vy, pz = s.vz;
-      while (dda_step<M>(s, r.L)) {
+      while (dda_step<M, true>(s, r.L)) {
         ++visits;  // miss on (px, py, pz)
         if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7) {
           c = cube_index(J.cube, s.vx, s.vy, s.vz);

@@ -55,12 +55,12 @@ NM_HD bool ray_from_point(const float* R, float px, float py, float pz,
   return true;
 }
 
-// bx ? a : (by ? b : c) on values, written with bit masks so the compiler
-// keeps it a register select.
-NM_HD double fmin_sel(bool bx, double a, bool by, double b, double c) {
-  const uint64_t mx = 0ull - (uint64_t)bx, my = 0ull - (uint64_t)(by && !bx);
-  const uint64_t mz = ~(mx | my);
-  return double_of((bits_of(a) & mx) | (bits_of(b) & my) | (bits_of(c) & mz));
+NM_HD double fmin_(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fmin(a, b);
+#else
+  return std::fmin(a, b);
+#endif
 }
 
 // Voxel DDA state.  Axes are kept in named scalars so the per-step
@@ -107,15 +107,36 @@ NM_HD void dda_init(const double* eye, const RayDir& r, Dda& s) {
 // axis choice is a select between register values (a select between two
 // struct fields can otherwise become a load from a selected address, which
 // pins the state to scratch memory).
-template <int M>
+//
+// Two equivalent forms of the axis choice, picked per kernel by
+// measurement (the discover pass runs faster with kMin, the accumulate
+// pass without it):
+//   kMin:  t = min(nx, ny, nz); the axis is z if nz is minimal, else y if
+//          ny is, else x (+0 and -0 compare equal here as in the rule);
+//   else:  x only if strictly smallest, else y if y < z, else z, with t
+//          and the increment selected by bit masks.
+template <int M, bool kMin = false>
 NM_HD bool dda_step(Dda& s, double L) {
   const double nx = s.nx, ny = s.ny, nz = s.nz;
   const double ddx = s.dx, ddy = s.dy, ddz = s.dz;
-  const bool bx = (nx < ny) && (nx < nz);
-  const bool by = !bx && (ny < nz);
-  const bool bz = !bx && !by;
-  const double t = fmin_sel(bx, nx, by, ny, nz);
-  const double dt = fmin_sel(bx, ddx, by, ddy, ddz);
+  bool bx, by, bz;
+  double t, dt;
+  if (kMin) {
+    t = fmin_(nx, fmin_(ny, nz));
+    bz = nz == t;
+    by = !bz && ny == t;
+    bx = !bz && !by;
+    dt = bz ? ddz : (by ? ddy : ddx);
+  } else {
+    bx = (nx < ny) && (nx < nz);
+    by = !bx && (ny < nz);
+    bz = !bx && !by;
+    const uint64_t mx = 0ull - (uint64_t)bx, my = 0ull - (uint64_t)by;
+    const uint64_t mz = ~(mx | my);
+    t = double_of((bits_of(nx) & mx) | (bits_of(ny) & my) | (bits_of(nz) & mz));
+    dt = double_of((bits_of(ddx) & mx) | (bits_of(ddy) & my) |
+                   (bits_of(ddz) & mz));
+  }
   // t >= 0 and dt > 0 (next times start at 0 + a non-negative product and
   // grow by |1/d|), so the round-down sum needs no sign or zero cases
   const double nt = M == kDown ? add_down_pos(t, dt) : t + dt;
