@@ -691,38 +691,110 @@ __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
 }
 
 // Pixel-order sum of one voxel's hit sequence, computed by a whole
-// wavefront: gap[g] misses precede hit g (g < j).  The wavefront loads the
-// gaps 64 at a time (the next chunk is in flight while one is summed); a
-// run of consecutive zero gaps is a run of hits, summed by one fold.  Every
-// lane returns the same value; *used = total of the gaps.
+// wavefront: gap[g] misses precede hit g (g < j); *used = total of the
+// gaps.  Lane i takes event i of a chunk of 64 (its gap run, then its
+// hit).  While the running sum v stays in one binade [2^e, 2^(e+1)) -- in
+// magnitude, same sign -- every value is a multiple of the ulp u and each
+// rounded step adds a fixed multiple of u: round(h / u) for a hit and
+// round(m / u) for a miss (rounded toward -inf, or to nearest when no tie
+// arises).  One exact integer prefix sum over the lanes then gives v after
+// every event, and the events are taken up to the first one whose run
+// leaves the binade's interior; that event is summed one step at a time
+// (fold), and the jump is tried again from there.  The result equals the
+// sequential sum bit for bit.
+__device__ __forceinline__ long long wave_incl_scan64(long long v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Increment of one event in units of the ulp (q = a / u exactly), or
+// false when it cannot be taken as a fixed step: larger than the binade,
+// or a round-to-nearest tie (whose result depends on the parity of v).
+template <int M>
+__device__ __forceinline__ bool unit_step(double q, long long* A) {
+  if (!(abs_(q) < 16777216.0)) return false;
+  if (M == kDown) {
+    *A = (long long)floor_(q);
+  } else {
+    if (q - floor_(q) == 0.5) return false;
+    *A = (long long)rint_(q);
+  }
+  return true;
+}
+
+// T (in ulps) lies strictly inside the binade: every value between T and
+// a neighbour inside it rounds on the same grid.
+__device__ __forceinline__ bool in_binade(long long T, bool neg) {
+  const long long m = neg ? -T : T;
+  return m >= (1ll << 23) + 1 && m <= (1ll << 24) - 1;
+}
+
 template <int M>
 __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
                                    uint32_t j, float miss, float hit,
                                    uint32_t* used_out) {
   const uint32_t lane = threadIdx.x & 63;
-  float acc = 0.0f;
+  float v = 0.0f;
   uint32_t used = 0;
   uint32_t cur = lane < j ? gap[lane] : 0u;
   for (uint32_t base = 0; base < j; base += 64) {
     const uint32_t n = j - base < 64u ? j - base : 64u;
-    const uint32_t nxt = base + 64 + lane < j ? gap[base + 64 + lane] : 0u;
-    uint64_t nzm = __ballot(cur != 0u);
-    uint32_t pos = 0;
-    while (nzm) {
-      const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
-      nzm &= nzm - 1;
-      acc = fold<M, true>(acc, hit, p - pos);
-      const uint32_t gp = (uint32_t)__shfl((int)cur, (int)p, 64);
-      acc = fold<M, true>(acc, miss, gp);
-      acc = add<M, true>(acc, hit);
-      used += gp;
-      pos = p + 1;
+    const uint32_t g = cur;
+    cur = base + 64 + lane < j ? gap[base + 64 + lane] : 0u;
+    uint32_t i0 = 0;
+    while (i0 < n) {
+      uint32_t f = 0;
+      const uint32_t ex = (bits_of(v) >> 23) & 0xffu;
+      long long Ah, Am;
+      if (ex != 0u && ex != 0xffu) {
+        const double inv_u = double_of((uint64_t)(1173u - ex) << 52);  // 2^-e'
+        if (unit_step<M>((double)hit * inv_u, &Ah) &&
+            unit_step<M>((double)miss * inv_u, &Am)) {
+          const long long S = (long long)((double)v * inv_u);  // exact
+          const bool neg = v < 0.0f;
+          const bool act = lane >= i0 && lane < n;
+          const long long P =
+              wave_incl_scan64(act ? (long long)g * Am + Ah : 0ll);
+          const uint32_t G = wave_incl_scan32(act ? g : 0u);
+          const bool ok = act && in_binade(S + P, neg) &&
+                          (g == 0u || in_binade(S + P - Ah, neg));
+          const uint64_t bad = ~__ballot(ok) & (~0ull << i0);
+          f = (bad ? (uint32_t)__builtin_ctzll(bad) : 64u) - i0;
+          if (f) {
+            const int last = (int)(i0 + f - 1);
+            const long long Pf = __shfl(P, last, 64);
+            const double u = double_of((uint64_t)(ex + 873u) << 52);
+            v = (float)((double)(S + Pf) * u);  // exact
+            used += (uint32_t)__shfl((int)G, last, 64);
+            i0 += f;
+          }
+        }
+      }
+      if (!f) {  // event i0 one step at a time
+        const uint32_t gi = (uint32_t)__shfl((int)g, (int)i0, 64);
+        v = fold<M, true>(v, miss, gi);
+        v = add<M, true>(v, hit);
+        used += gi;
+        ++i0;
+      }
     }
-    acc = fold<M, true>(acc, hit, n - pos);
-    cur = nxt;
   }
   *used_out = used;
-  return acc;
+  return v;
 }
 
 // Bits of a tile's hit word after k_temp: the voxel was touched, and it
