@@ -270,3 +270,41 @@ def test_bad_arguments_raise(gpu):
     with pytest.raises(nm.NanoMapError):
         ctx.cast(f, np.ones((10, 3), np.float32))  # previous cast not folded
     ctx.update(g)
+
+
+def _near_wall_cloud(w, h, seed):
+    """A w x h depth image of a tilted plane 0.4-0.9 m in front of the
+    sensor (optical frame: z forward), with depth noise and dropouts: 0.2 m
+    voxels on the plane collect thousands of hits each, and rays that end
+    in front of a voxel or pass its corner leave misses between them."""
+    rng = np.random.default_rng(seed)
+    u = (np.arange(w) + 0.5) / w * 2 - 1
+    v = (np.arange(h) + 0.5) / h * 2 - 1
+    U, V = np.meshgrid(u * np.tan(np.radians(43.5)), v * np.tan(np.radians(29)))
+    z = 0.4 + 0.25 * (U + 1) + 0.1 * (V + 1)
+    z = z * (1 + rng.normal(0, 0.02, z.shape))
+    pts = np.stack([U * z, V * z, z], -1).reshape(-1, 3).astype(np.float32)
+    pts[rng.random(len(pts)) < 0.03] = np.nan
+    return pts
+
+
+@pytest.mark.parametrize("rounding", [RD, RN])
+def test_many_hit_voxels_bit_exact(gpu, rounding):
+    """Voxels with hundreds to thousands of hits in one frame, summed by the
+    wavefront-parallel path, against the oracle's sequential sums."""
+    nm = gpu
+    w, h = 320, 240
+    F = np.array([[0, 0, 1], [-1, 0, 0], [0, -1, 0]], np.float32)
+    ctx = nm.Context(w * h, 5.0 / 0.2 + 1)
+    gmap = nm.Map(0.2, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for k in range(2):
+        f = nm.frame_from_pose([1.03 + 0.07 * k, 2.11, 0.97], [1, 0, 0, 0], F,
+                               0.2, 5.0, 0.85, -0.4, rounding, w, h)
+        temp = _run_frame(nm, ctx, gmap, omap, f, _near_wall_cloud(w, h, k))
+        _, m, hh, _ = temp.export()
+        print("hits per hit voxel: max %d, voxels > 256 hits: %d, mixed: %d"
+              % (hh.max(), (hh > 256).sum(), ((hh > 256) & (m > 0)).sum()),
+              flush=True)
+        assert (hh > 256).sum() >= 10 and ((hh > 256) & (m > 0)).sum() >= 3
+    _check_maps(gmap, omap)
