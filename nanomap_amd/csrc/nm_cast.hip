@@ -433,22 +433,38 @@ __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
     atomicAdd(&tiles[2 * (size_t)cell], inc);
 }
 
-// lds_count for the active lanes; when they all name the same voxel (the
-// voxels next to the eye) one lane adds the lane count.
+// lds_count for the active lanes.  Neighbouring rays at the same step
+// mostly stand in the same voxel (an 8x8 pixel wavefront covers a few
+// voxels at any depth), so lanes are merged before the table is touched:
+// when they all name the same voxel (the voxels next to the eye) one lane
+// adds the lane count; otherwise each run of equal cells along an 8-lane
+// pixel row (lane - 1 read across the row with a DPP shift) is added once,
+// by its first lane, with the run's length.
 template <int S>
 __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
                                                uint32_t* s_cnt,
                                                uint32_t* tiles,
                                                uint32_t cell) {
   const uint64_t act = __ballot(1);
+  const int lane = threadIdx.x & 63;
   const uint32_t first = __builtin_amdgcn_readfirstlane(cell);
   const uint64_t same = __ballot(cell == first);
   if (same == act) {
-    const int lane = threadIdx.x & 63;
     if (lane == (int)__builtin_ctzll(act))
       lds_count<S>(s_key, s_cnt, tiles, cell, (uint32_t)__popcll(act));
-  } else {
-    lds_count<S>(s_key, s_cnt, tiles, cell, 1u);
+    return;
+  }
+  // row_shr:1 -- lane l reads lane l - 1 of its 16-lane row
+  const uint32_t left =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)cell, 0x111, 0xf, 0xf, false);
+  const bool left_act = (lane & 7) != 0 && ((act >> (lane - 1)) & 1ull);
+  const bool head = !left_act || left != cell;
+  const uint64_t stop = __ballot(head) | ~act | 0x0101010101010101ull;
+  if (head) {
+    const uint64_t after = lane == 63 ? 0ull : (stop >> (lane + 1));
+    const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u
+                               : (uint32_t)(64 - lane);
+    lds_count<S>(s_key, s_cnt, tiles, cell, len);
   }
 }
 
