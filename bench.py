@@ -42,6 +42,9 @@ def parse():
                     help="timed passes of the 32-agent batch (0 = skip)")
     ap.add_argument("--soft-steps", type=int, default=10,
                     help="timed passes of the soft march (0 = skip)")
+    ap.add_argument("--only", choices=["all", "batch", "soft"], default="all",
+                    help="run one part only (for profiling): the 32-agent "
+                         "batch or the soft march; prints that part's JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed steps, run one fill and one "
                          "elementwise copy of this many MiB (the known-size "
@@ -297,6 +300,15 @@ def main():
     from nanomap_amd import workloads as W
 
     rounding = nm.ROUND_DOWN if args.rounding == "down" else nm.ROUND_NEAREST
+    if args.only == "batch":
+        print(json.dumps({"batched": bench_batch(nm, W, torch,
+                                                 max(args.batch_steps, 1),
+                                                 rounding)}))
+        return
+    if args.only == "soft":
+        print(json.dumps({"soft_march": bench_soft(nm, W, torch,
+                                                   max(args.soft_steps, 1))}))
+        return
     t_setup = time.perf_counter()
     scene, cam, frames = W.cfg2_frames(nm, n_frames=args.frames)
     P = cam.width * cam.height

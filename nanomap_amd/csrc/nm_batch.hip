@@ -261,6 +261,7 @@ int build_layout(nm_batch* b, const nm_agent_frame* f, int n) {
   p.nitems = (int)item_off;
   p.tile_cap = b->dims.tile_cap;
   p.sort_bits = nm::sort_bits_for((int64_t)b->dims.tile_cap * 512);
+  p.heavy_hits = nm::kBatchHeavyHits;
   p.rounding = b->rounding;
   p.cu_count = b->cu_count;
   p.tuning = b->tuning;

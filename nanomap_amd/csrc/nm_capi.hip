@@ -437,6 +437,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   pl.nitems = p.nitems;
   pl.tile_cap = ctx->tile_cap;
   pl.sort_bits = nm::sort_bits_for((int64_t)ctx->tile_cap * 512);
+  pl.heavy_hits = nm::kFrameHeavyHits;
   pl.rounding = frame->rounding;
   pl.cu_count = ctx->cu_count;
   ctx->job = p;

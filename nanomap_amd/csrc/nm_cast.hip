@@ -716,8 +716,10 @@ __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
 // arises).  One exact integer prefix sum over the lanes then gives v after
 // every event, and the events are taken up to the first one whose run
 // leaves the binade's interior; that event is summed one step at a time
-// (fold), and the jump is tried again from there.  The result equals the
-// sequential sum bit for bit.
+// (fold), and the jump is tried again from there (near zero, where a
+// binade holds few events, events are summed one at a time).  A chunk with
+// few nonzero gaps is summed as runs of hits instead: one fold per run.
+// The result equals the sequential sum bit for bit.
 __device__ __forceinline__ long long wave_incl_scan64(long long v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -764,6 +766,8 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
                                    uint32_t j, float miss, float hit,
                                    uint32_t* used_out) {
   const uint32_t lane = threadIdx.x & 63;
+  // below this magnitude a binade holds only a few events: single steps
+  const float small = 8.0f * fmaxf(fabsf(hit), fabsf(miss));
   float v = 0.0f;
   uint32_t used = 0;
   uint32_t cur = lane < j ? gap[lane] : 0u;
@@ -771,12 +775,29 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     const uint32_t n = j - base < 64u ? j - base : 64u;
     const uint32_t g = cur;
     cur = base + 64 + lane < j ? gap[base + 64 + lane] : 0u;
+    uint64_t nzm = __ballot(g != 0u && lane < n);
+    if (__popcll(nzm) <= 8) {
+      // few misses in this chunk: runs of hits between them, one fold each
+      uint32_t pos = 0;
+      while (nzm) {
+        const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
+        nzm &= nzm - 1;
+        const uint32_t gp = (uint32_t)__shfl((int)g, (int)p, 64);
+        v = fold<M, true>(v, hit, p - pos);
+        v = fold<M, true>(v, miss, gp);
+        v = add<M, true>(v, hit);
+        used += gp;
+        pos = p + 1;
+      }
+      v = fold<M, true>(v, hit, n - pos);
+      continue;
+    }
     uint32_t i0 = 0;
     while (i0 < n) {
       uint32_t f = 0;
       const uint32_t ex = (bits_of(v) >> 23) & 0xffu;
       long long Ah, Am;
-      if (ex != 0u && ex != 0xffu) {
+      if (ex != 0u && ex != 0xffu && !(fabsf(v) < small)) {
         const double inv_u = double_of((uint64_t)(1173u - ex) << 52);  // 2^-e'
         if (unit_step<M>((double)hit * inv_u, &Ah) &&
             unit_step<M>((double)miss * inv_u, &Am)) {
@@ -830,7 +851,7 @@ __global__ __launch_bounds__(256) void k_temp(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ gap, FrameCounters* __restrict__ cnt,
-    uint32_t* __restrict__ heavy_list, int tile_cap) {
+    uint32_t* __restrict__ heavy_list, int tile_cap, uint32_t heavy_hits) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t nvox = (int64_t)min(cnt->n_leaves, tile_cap) * 512;
@@ -847,7 +868,7 @@ __global__ __launch_bounds__(256) void k_temp(
     const uint32_t s0 = end ? hit_start[cell] : 0u;
     const uint32_t j = end - s0;
     const bool touched = (k | j) != 0;
-    const bool heavy = j > kHeavyHits;
+    const bool heavy = j > heavy_hits;
     const uint64_t hv = __ballot(heavy);
     if (hv) {
       int first = 0;
@@ -861,16 +882,16 @@ __global__ __launch_bounds__(256) void k_temp(
     if (j == 0) {
       tv = fold<M, true>(0.0f, miss, k);
     } else {
-      // gaps are loaded 4 at a time so the loads overlap
+      // gaps are loaded 8 at a time so the loads overlap
       uint32_t used = 0;
       float acc = 0.0f;
-      for (uint32_t g = 0; g < j; g += 4) {
-        uint32_t gm[4];
+      for (uint32_t g = 0; g < j; g += 8) {
+        uint32_t gm[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 8; ++u)
           gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
           if (g + u < j) {
             if (gm[u] != 0u) {
               acc = fold_small<M>(acc, miss, gm[u]);
@@ -1225,7 +1246,7 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   if (p.rounding == kDown) {
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                         b.hit_start, b.gap, b.counters,
-                                        b.heavy, p.tile_cap);
+                                        b.heavy, p.tile_cap, p.heavy_hits);
     k_heavy<kDown><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                          b.hit_start, b.gap, b.counters,
                                          b.heavy);
@@ -1236,7 +1257,7 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   } else {
     k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                            b.hit_start, b.gap, b.counters,
-                                           b.heavy, p.tile_cap);
+                                           b.heavy, p.tile_cap, p.heavy_hits);
     k_heavy<kNearest><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                             b.hit_start, b.gap, b.counters,
                                             b.heavy);

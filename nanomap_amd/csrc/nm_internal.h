@@ -136,8 +136,8 @@ struct Workspace {
   uint32_t* keys[2];    // [ray_cap]  radix sort buffers
   uint32_t* vals[2];
   uint32_t* gap;        // [ray_cap]  misses before each sorted hit
-  uint32_t* heavy;      // [ray_cap / kHeavyHits + 1]  cells of voxels with
-                        //             more than kHeavyHits hits
+  uint32_t* heavy;      // [ray_cap / kMinHeavyHits + 1]  cells of voxels
+                        //             with more than heavy_hits hits
   uint32_t* radix_hist; // radix_hist_words(ray_cap)
   int32_t* scan_blocks; // [cube_cap / kScanChunk + 2]
   int32_t* scan_job;    // [cube_cap / kScanChunk + 2]  job of each chunk
@@ -174,8 +174,14 @@ constexpr int kScanChunk = 4096;
 constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
-// Voxels with more hits than this in one frame are summed by a wavefront.
-constexpr uint32_t kHeavyHits = 256;
+// Voxels with more hits than Pipeline::heavy_hits (at least kMinHeavyHits)
+// in one launch sequence are summed by a whole wavefront each.  A single
+// frame is latency-bound and lists voxels from 16 hits on (no lane sums a
+// long sequence alone); a batch is throughput-bound and lists them from
+// 256 (a lane per voxel uses the machine better).
+constexpr uint32_t kMinHeavyHits = 16;
+constexpr uint32_t kFrameHeavyHits = 16;
+constexpr uint32_t kBatchHeavyHits = 256;
 
 // ---- launchers (nm_cast.hip, nm_radix.hip, nm_filter.hip) -------------
 // The frame sequence over a device job table `b.jobs[0, njobs)`:
@@ -192,6 +198,7 @@ struct Pipeline {
   int nitems;         // cast work items of all jobs
   int tile_cap;
   int sort_bits;
+  uint32_t heavy_hits = kFrameHeavyHits;  // see kMinHeavyHits
   int rounding;
   int cu_count;
   Tuning tuning;

@@ -40,7 +40,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.vals[0], P));
   A(dalloc(&b.vals[1], P));
   A(dalloc(&b.gap, P));
-  A(dalloc(&b.heavy, P / kHeavyHits + 1));
+  A(dalloc(&b.heavy, P / kMinHeavyHits + 1));
   A(dalloc(&b.radix_hist, (size_t)radix_hist_words((int)P)));
   A(dalloc(&b.scan_blocks, scan_blocks));
   A(dalloc(&b.scan_job, scan_blocks));
