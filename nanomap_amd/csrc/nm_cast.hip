@@ -380,13 +380,16 @@ __global__ __launch_bounds__(256) void k_hit_segments(
     // start (the hit count is end - start)
     if (e + 1 == nh || keys[e + 1] != key)
       tiles[2 * (size_t)key + 1] = (uint32_t)(e + 1);
-    if (e > 0 && keys[e - 1] == key) continue;
+    const uint32_t prev = e > 0 ? keys[e - 1] : kInvalid;
+    if (prev == key) continue;
     hit_start[key] = (uint32_t)e;
     const uint32_t slot = key >> 9, li = key & 511;
     atomicOr(&hit_mask[slot * kMaskWords + (li >> 5)], 1u << (li & 31));
-    // flag the leaf in the cube table so the accumulate pass reads its hit
-    // mask only inside leaves that hold hits
-    atomicOr(&leaf_slot[slot_leaf[slot]], kSlotHasHits);
+    // the leaf's first hit voxel flags the leaf in the cube table (sole
+    // writer: keys are sorted leaf-major), so the accumulate pass reads
+    // its hit mask only inside leaves that hold hits
+    if (e == 0 || (prev >> 9) != slot)
+      leaf_slot[slot_leaf[slot]] = (int32_t)slot | kSlotHasHits;
   }
 }
 
