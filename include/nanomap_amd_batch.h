@@ -28,7 +28,7 @@ typedef struct nm_batch_config {
   int max_frames;         /* frames per batch                               */
   int64_t max_points;     /* points of all frames of a batch                */
   float max_time;         /* largest per-frame max_time (voxels)            */
-  int64_t tile_capacity;  /* leaf tiles of all frames of a batch            */
+  int64_t tile_capacity;  /* leaf tiles of all frames of a batch, <= 2^22   */
 } nm_batch_config;
 
 /* One frame of one agent.  The cloud is device-resident: xyz f32 at byte

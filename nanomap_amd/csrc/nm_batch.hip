@@ -358,7 +358,7 @@ int nm_batch_create(const nm_batch_config* cfg, nm_batch** out) {
   if (cfg->max_frames <= 0 || cfg->max_points <= 0 ||
       cfg->max_points > ((int64_t)1 << 31) - 1 || !(cfg->max_time > 0) ||
       cfg->max_time > 4000 || cfg->tile_capacity <= 0 ||
-      cfg->tile_capacity > (1 << 23))
+      cfg->tile_capacity > (1 << 22))
     return bfail(NM_EINVAL, "nm_batch_create: bad configuration");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)

@@ -393,8 +393,10 @@ __global__ __launch_bounds__(256) void k_hit_segments(
   }
 }
 
-// LDS aggregation of miss counts: an open-addressing table of (cell,
-// count) per workgroup.  A 16x16 pixel tile touches a few thousand distinct
+// LDS aggregation of miss counts: an open-addressing table of (key,
+// count) per workgroup.  A key is a voxel cell (slot * 512 + local index,
+// below 2^31), whose count goes to its miss counter, or kGapKey | i, whose
+// count goes to gap[i] (misses of a hit voxel before its i-th sorted hit).  A 16x16 pixel tile touches a few thousand distinct
 // voxels for tens of thousands of visits, so the table turns almost every
 // visit into an LDS atomic and leaves one global atomic per distinct voxel
 // per work item, at flush.  A visit that finds no free slot within kProbe
@@ -405,11 +407,18 @@ __global__ __launch_bounds__(256) void k_hit_segments(
 // flight, can prefer 2048 slots for the extra resident workgroups -- the
 // table size is a template parameter (S = log2 slots).
 constexpr int kProbe = 2;
+constexpr uint32_t kGapKey = 0x80000000u;
+
+// The global counter of a table key.
+__device__ __forceinline__ uint32_t* key_counter(uint32_t* tiles,
+                                                 uint32_t* gap, uint32_t key) {
+  return (key & kGapKey) ? gap + (key & ~kGapKey) : tiles + 2 * (size_t)key;
+}
 
 template <int S>
 __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
-                                          uint32_t* tiles, uint32_t cell,
-                                          uint32_t inc) {
+                                          uint32_t* tiles, uint32_t* gap,
+                                          uint32_t cell, uint32_t inc) {
   uint32_t h = (cell * 2654435761u) >> (32 - S);  // multiplicative hash
   int found = -1;
 #pragma unroll
@@ -433,7 +442,7 @@ __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
     __hip_atomic_fetch_add(&s_cnt[found], inc, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
   else
-    atomicAdd(&tiles[2 * (size_t)cell], inc);
+    atomicAdd(key_counter(tiles, gap, cell), inc);
 }
 
 // lds_count for the active lanes.  Neighbouring rays at the same step
@@ -447,6 +456,7 @@ template <int S>
 __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
                                                uint32_t* s_cnt,
                                                uint32_t* tiles,
+                                               uint32_t* gap,
                                                uint32_t cell) {
   const uint64_t act = __ballot(1);
   const int lane = threadIdx.x & 63;
@@ -454,7 +464,7 @@ __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
   const uint64_t same = __ballot(cell == first);
   if (same == act) {
     if (lane == (int)__builtin_ctzll(act))
-      lds_count<S>(s_key, s_cnt, tiles, cell, (uint32_t)__popcll(act));
+      lds_count<S>(s_key, s_cnt, tiles, gap, cell, (uint32_t)__popcll(act));
     return;
   }
   // row_shr:1 -- lane l reads lane l - 1 of its 16-lane row
@@ -467,7 +477,7 @@ __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
     const uint64_t after = lane == 63 ? 0ull : (stop >> (lane + 1));
     const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u
                                : (uint32_t)(64 - lane);
-    lds_count<S>(s_key, s_cnt, tiles, cell, len);
+    lds_count<S>(s_key, s_cnt, tiles, gap, cell, len);
   }
 }
 
@@ -523,10 +533,12 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
             const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
             hit_voxel = (mw >> (li & 31)) & 1u;
           }
+          wave_lds_count<S>(s_key, s_cnt, tiles, gap, cell);
           if (hit_voxel) {
-            // a voxel holding hits: count the miss directly and mark the
-            // gap before the first later hit of this voxel (its hits are
-            // sorted[s0, end) in ascending point order)
+            // a voxel holding hits also counts the miss in the gap before
+            // its first later hit (the voxel's hits are sorted[s0, end) in
+            // ascending point order); neighbouring rays mostly share the
+            // gap, so these counts are merged in the table as well
             const uint32_t s0 = hit_start[cell];
             const uint32_t end = tiles[2 * (size_t)cell + 1];
             uint32_t lo = s0, hi = end;
@@ -537,10 +549,8 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
               else
                 hi = mid;
             }
-            if (lo < end) atomicAdd(&gap[lo], 1u);
-            atomicAdd(&tiles[2 * (size_t)cell], 1u);
-          } else {
-            wave_lds_count<S>(s_key, s_cnt, tiles, cell);
+            if (lo < end)
+              wave_lds_count<S>(s_key, s_cnt, tiles, gap, kGapKey | lo);
           }
         }
         if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     for (int k = threadIdx.x; k < kSlots; k += kCastThreads) {
       const uint32_t key = s_key[k];
       if (key != kInvalid) {
-        atomicAdd(&tiles[2 * (size_t)key], s_cnt[k]);
+        atomicAdd(key_counter(tiles, gap, key), s_cnt[k]);
         s_key[k] = kInvalid;
         s_cnt[k] = 0;
       }
@@ -723,22 +733,17 @@ __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
 // binade holds few events, events are summed one at a time).  A chunk with
 // few nonzero gaps is summed as runs of hits instead: one fold per run.
 // The result equals the sequential sum bit for bit.
-__device__ __forceinline__ long long wave_incl_scan64(long long v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const long long t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
+// Inclusive prefix sum over the 64 lanes of a fully active wavefront, in
+// DPP moves: lanes read lane - 1, 2, 4, 8 inside their 16-lane row (0 past
+// the row start), then lane 15 of each row is added to the next row (rows
+// 1 and 3) and lane 31 to rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
   return v;
 }
 
@@ -773,11 +778,21 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
   const float small = 8.0f * fmaxf(fabsf(hit), fabsf(miss));
   float v = 0.0f;
   uint32_t used = 0;
-  uint32_t cur = lane < j ? gap[lane] : 0u;
+  // gaps are loaded kGroup chunks at a time, so one memory latency is
+  // spread over kGroup * 64 events
+  constexpr uint32_t kGroup = 8;
+  uint32_t grp[kGroup];
   for (uint32_t base = 0; base < j; base += 64) {
+    const uint32_t c = (base >> 6) & (kGroup - 1);
+    if (c == 0) {
+#pragma unroll
+      for (uint32_t t = 0; t < kGroup; ++t) {
+        const uint32_t e = base + t * 64 + lane;
+        grp[t] = e < j ? gap[e] : 0u;
+      }
+    }
     const uint32_t n = j - base < 64u ? j - base : 64u;
-    const uint32_t g = cur;
-    cur = base + 64 + lane < j ? gap[base + 64 + lane] : 0u;
+    const uint32_t g = grp[c];
     uint64_t nzm = __ballot(g != 0u && lane < n);
     if (__popcll(nzm) <= 8) {
       // few misses in this chunk: runs of hits between them, one fold each
@@ -807,9 +822,11 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
           const long long S = (long long)((double)v * inv_u);  // exact
           const bool neg = v < 0.0f;
           const bool act = lane >= i0 && lane < n;
-          const long long P =
-              wave_incl_scan64(act ? (long long)g * Am + Ah : 0ll);
+          // the sum of (g Am + Ah) over events i0..lane, from the prefix
+          // sum G of the gaps (below 2^31: they count one voxel's misses)
           const uint32_t G = wave_incl_scan32(act ? g : 0u);
+          const long long P =
+              (long long)G * Am + (long long)((int)lane - (int)i0 + 1) * Ah;
           const bool ok = act && in_binade(S + P, neg) &&
                           (g == 0u || in_binade(S + P - Ah, neg));
           const uint64_t bad = ~__ballot(ok) & (~0ull << i0);
