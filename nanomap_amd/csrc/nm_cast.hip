@@ -730,9 +730,10 @@ __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
 // every event, and the events are taken up to the first one whose run
 // leaves the binade's interior; that event is summed one step at a time
 // (fold), and the jump is tried again from there (near zero, where a
-// binade holds few events, events are summed one at a time).  A chunk with
-// few nonzero gaps is summed as runs of hits instead: one fold per run.
-// The result equals the sequential sum bit for bit.
+// binade holds few events, events are summed one at a time).  Near zero, a
+// chunk with few nonzero gaps is summed as runs of hits instead: one fold
+// per run.  The result equals the sequential sum bit for bit.
+
 // Inclusive prefix sum over the 64 lanes of a fully active wavefront, in
 // DPP moves: lanes read lane - 1, 2, 4, 8 inside their 16-lane row (0 past
 // the row start), then lane 15 of each row is added to the next row (rows
@@ -794,8 +795,10 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     const uint32_t n = j - base < 64u ? j - base : 64u;
     const uint32_t g = grp[c];
     uint64_t nzm = __ballot(g != 0u && lane < n);
-    if (__popcll(nzm) <= 8) {
-      // few misses in this chunk: runs of hits between them, one fold each
+    if (__popcll(nzm) <= 8 && fabsf(v) < small) {
+      // near zero (where the prefix sums below step one event at a time)
+      // and few misses in this chunk: runs of hits between them, one fold
+      // each
       uint32_t pos = 0;
       while (nzm) {
         const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
