@@ -779,6 +779,18 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
   const float small = 8.0f * fmaxf(fabsf(hit), fabsf(miss));
   float v = 0.0f;
   uint32_t used = 0;
+  // Inside a binade the running sum is kept as S ulps (binade exponent ex,
+  // steps Ah and Am ulps); v is brought up to date when the binade is left.
+  bool in = false;
+  uint32_t ex = 0;
+  long long S = 0, Ah = 0, Am = 0;
+  auto leave = [&]() {
+    if (in) {
+      const double u = double_of((uint64_t)(ex + 873u) << 52);
+      v = (float)((double)S * u);  // exact
+      in = false;
+    }
+  };
   // gaps are loaded kGroup chunks at a time, so one memory latency is
   // spread over kGroup * 64 events
   constexpr uint32_t kGroup = 8;
@@ -795,7 +807,7 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     const uint32_t n = j - base < 64u ? j - base : 64u;
     const uint32_t g = grp[c];
     uint64_t nzm = __ballot(g != 0u && lane < n);
-    if (__popcll(nzm) <= 8 && fabsf(v) < small) {
+    if (!in && __popcll(nzm) <= 8 && fabsf(v) < small) {
       // near zero (where the prefix sums below step one event at a time)
       // and few misses in this chunk: runs of hits between them, one fold
       // each
@@ -803,7 +815,7 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
       while (nzm) {
         const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
         nzm &= nzm - 1;
-        const uint32_t gp = (uint32_t)__shfl((int)g, (int)p, 64);
+        const uint32_t gp = (uint32_t)__builtin_amdgcn_readlane((int)g, (int)p);
         v = fold<M, true>(v, hit, p - pos);
         v = fold<M, true>(v, miss, gp);
         v = add<M, true>(v, hit);
@@ -815,37 +827,42 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     }
     uint32_t i0 = 0;
     while (i0 < n) {
-      uint32_t f = 0;
-      const uint32_t ex = (bits_of(v) >> 23) & 0xffu;
-      long long Ah, Am;
-      if (ex != 0u && ex != 0xffu && !(fabsf(v) < small)) {
-        const double inv_u = double_of((uint64_t)(1173u - ex) << 52);  // 2^-e'
-        if (unit_step<M>((double)hit * inv_u, &Ah) &&
-            unit_step<M>((double)miss * inv_u, &Am)) {
-          const long long S = (long long)((double)v * inv_u);  // exact
-          const bool neg = v < 0.0f;
-          const bool act = lane >= i0 && lane < n;
-          // the sum of (g Am + Ah) over events i0..lane, from the prefix
-          // sum G of the gaps (below 2^31: they count one voxel's misses)
-          const uint32_t G = wave_incl_scan32(act ? g : 0u);
-          const long long P =
-              (long long)G * Am + (long long)((int)lane - (int)i0 + 1) * Ah;
-          const bool ok = act && in_binade(S + P, neg) &&
-                          (g == 0u || in_binade(S + P - Ah, neg));
-          const uint64_t bad = ~__ballot(ok) & (~0ull << i0);
-          f = (bad ? (uint32_t)__builtin_ctzll(bad) : 64u) - i0;
-          if (f) {
-            const int last = (int)(i0 + f - 1);
-            const long long Pf = __shfl(P, last, 64);
-            const double u = double_of((uint64_t)(ex + 873u) << 52);
-            v = (float)((double)(S + Pf) * u);  // exact
-            used += (uint32_t)__shfl((int)G, last, 64);
-            i0 += f;
+      if (!in) {
+        ex = (bits_of(v) >> 23) & 0xffu;
+        if (ex != 0u && ex != 0xffu && !(fabsf(v) < small)) {
+          const double inv_u = double_of((uint64_t)(1173u - ex) << 52);  // 2^-e'
+          if (unit_step<M>((double)hit * inv_u, &Ah) &&
+              unit_step<M>((double)miss * inv_u, &Am)) {
+            S = (long long)((double)v * inv_u);  // exact
+            in = true;
           }
         }
       }
-      if (!f) {  // event i0 one step at a time
-        const uint32_t gi = (uint32_t)__shfl((int)g, (int)i0, 64);
+      uint32_t f = 0;
+      if (in) {
+        const bool neg = S < 0;
+        const bool act = lane >= i0 && lane < n;
+        // the sum of (g Am + Ah) over events i0..lane, from the prefix
+        // sum G of the gaps (below 2^31: they count one voxel's misses)
+        const uint32_t G = wave_incl_scan32(act ? g : 0u);
+        const long long P =
+            (long long)G * Am + (long long)((int)lane - (int)i0 + 1) * Ah;
+        const bool ok = act && in_binade(S + P, neg) &&
+                        (g == 0u || in_binade(S + P - Ah, neg));
+        const uint64_t bad = ~__ballot(ok) & (~0ull << i0);
+        f = (bad ? (uint32_t)__builtin_ctzll(bad) : 64u) - i0;
+        if (f) {
+          const int last = (int)(i0 + f - 1);
+          const uint32_t Gf = (uint32_t)__builtin_amdgcn_readlane((int)G, last);
+          S += (long long)Gf * Am + (long long)f * Ah;
+          used += Gf;
+          i0 += f;
+        }
+      }
+      if (i0 < n) {
+        // event i0 leaves the binade (or no jump applies): one step
+        leave();
+        const uint32_t gi = (uint32_t)__builtin_amdgcn_readlane((int)g, (int)i0);
         v = fold<M, true>(v, miss, gi);
         v = add<M, true>(v, hit);
         used += gi;
@@ -853,6 +870,7 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
       }
     }
   }
+  leave();
   *used_out = used;
   return v;
 }
@@ -1265,7 +1283,9 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
                                    single, p.tile_cap, b.counters, b.map_idx,
                                    b.map_new);
   const int tgrid = p.cu_count * 8;
-  const int hgrid = p.cu_count * 2;
+  // one wavefront per heavy voxel and eight per SIMD to hide the latency
+  // of its serial chain (the count is on the device: surplus waves exit)
+  const int hgrid = p.cu_count * 8;
   if (p.rounding == kDown) {
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                         b.hit_start, b.gap, b.counters,
