@@ -180,6 +180,81 @@ __device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
                        s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
 }
 
+// Leaf-level form of the voxel DDA for the discover pass.  Each axis's
+// crossing times form their own sequence, c(k+1) = round(c(k) + delta),
+// independent of the other axes: the DDA only merges them in time order
+// (ties: z, then y, then x) and stops at the first crossing after L.  The
+// leaf changes only at every eighth crossing of an axis, so the pass merges
+// just those leaf-entering crossings and produces each axis's times eight
+// at a time.  Inside one binade [2^e, 2^(e+1)) every time is a multiple of
+// its ulp u and a rounded step adds the fixed D = round(delta / u) u, so
+// eight steps that stay in the binade (and at or below L) are one exact
+// jump; elsewhere the steps are taken one by one.  The visit count is the
+// number of crossings at or below L, the end voxel is the start voxel moved
+// by that many crossings per axis, and the exit time is the first crossing
+// above L.
+struct AxisRun {
+  double t;     // time of the pending leaf-entering crossing
+  double d;     // time between crossings
+  double over;  // first crossing time above L (DBL_MAX until produced)
+  int n;        // crossings at or below L produced so far
+};
+
+template <int M>
+__device__ __forceinline__ void axis_steps(AxisRun& a, int k, double L) {
+  for (int i = 0; i < k; ++i) {
+    a.t = M == kDown ? add_down_pos(a.t, a.d) : a.t + a.d;
+    if (a.t <= L)
+      ++a.n;
+    else if (a.over == 1.7976931348623157e308)
+      a.over = a.t;
+  }
+}
+
+// The next leaf-entering crossing: eight crossings on.
+template <int M>
+__device__ __forceinline__ void axis_leaf(AxisRun& a, double L) {
+  const uint32_t e = (uint32_t)(bits_of(a.t) >> 52);  // t >= 0: exponent
+  if (e > 52u && e < 0x7feu) {
+    const double u = double_of((uint64_t)(e - 52u) << 52);
+    const double top = double_of((uint64_t)(e + 1u) << 52);
+    const double q = a.d * double_of((uint64_t)(2098u - e) << 52);  // d / u
+    const double fq = floor_(q);
+    // to nearest, a tie's step depends on the parity of t: no jump
+    if (M == kDown || q - fq != 0.5) {
+      const double D = (M == kDown ? fq : rint_(q)) * u;  // exact
+      // every partial sum exact below top; the eighth step's sum below top
+      const double t7 = a.t + 7.0 * D;
+      if (D > 0.0 && t7 < top && top - t7 > a.d && t7 + D <= L) {
+        a.t = t7 + D;
+        a.n += 8;
+        return;
+      }
+    }
+  }
+  axis_steps<M>(a, 8, L);
+}
+
+// First leaf-entering crossing of an axis from its first crossing time c1:
+// crossing k moves the voxel from v0 + s(k - 1) to v0 + s k.
+template <int M>
+__device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
+                                              int v0, double L) {
+  AxisRun a;
+  a.d = d;
+  a.over = 1.7976931348623157e308;
+  a.n = 0;
+  a.t = c1;
+  if (s == 0) return a;  // never crosses (time DBL_MAX)
+  if (c1 <= L)
+    a.n = 1;
+  else
+    a.over = c1;
+  const int k0 = s > 0 ? 8 - (v0 & 7) : (v0 & 7) + 1;
+  axis_steps<M>(a, k0 - 1, L);
+  return a;
+}
+
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
     const Job* __restrict__ jobs, ItemQueue q, int32_t* __restrict__ leaf_slot,
@@ -199,30 +274,44 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
       valid = 1;
       Dda s;
       dda_init<M>(J.eye, r, s);
-      int c = cube_index(J.cube, s.vx, s.vy, s.vz);
+      const double L = r.L;
+      AxisRun ax = axis_start<M>(s.nx, s.dx, s.sx, s.vx, L);
+      AxisRun ay = axis_start<M>(s.ny, s.dy, s.sy, s.vy, L);
+      AxisRun az = axis_start<M>(s.nz, s.dz, s.sz, s.vz, L);
+      // a voxel of the current leaf per axis (only v >> 3 is used)
+      int lx = s.vx, ly = s.vy, lz = s.vz;
+      int c = cube_index(J.cube, lx, ly, lz);
       if (c < 0)
         ++errors;
       else
         ls[c] = -2;
-      int px = s.vx, py = s.vy, pz = s.vz;
-      while (dda_step<M, true>(s, r.L)) {
-        ++visits;  // miss on (px, py, pz)
-        if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7) {
-          c = cube_index(J.cube, s.vx, s.vy, s.vz);
-          if (c < 0)
-            ++errors;
-          else
-            ls[c] = -2;
+      for (;;) {
+        const double t = fmin_(ax.t, fmin_(ay.t, az.t));
+        if (!(t <= L)) break;
+        if (az.t == t) {
+          lz += 8 * s.sz;
+          axis_leaf<M>(az, L);
+        } else if (ay.t == t) {
+          ly += 8 * s.sy;
+          axis_leaf<M>(ay, L);
+        } else {
+          lx += 8 * s.sx;
+          axis_leaf<M>(ax, L);
         }
-        px = s.vx;
-        py = s.vy;
-        pz = s.vz;
+        c = cube_index(J.cube, lx, ly, lz);
+        if (c < 0)
+          ++errors;
+        else
+          ls[c] = -2;
       }
-      if (s.t < (double)J.max_time) {
+      visits = (uint32_t)(ax.n + ay.n + az.n);
+      if (fmin_(ax.over, fmin_(ay.over, az.over)) < (double)J.max_time) {
         ++visits;
-        const int ce = cube_index(J.cube, px, py, pz);
+        const int ex = s.vx + s.sx * ax.n, ey = s.vy + s.sy * ay.n,
+                  ez = s.vz + s.sz * az.n;
+        const int ce = cube_index(J.cube, ex, ey, ez);
         if (ce >= 0)
-          hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(px, py, pz);
+          hit_cell = (uint32_t)ce * 512u + (uint32_t)local_of(ex, ey, ez);
       }
     }
     if (i >= 0) ray_hit[(size_t)J.ray_off + i] = hit_cell;
@@ -770,6 +859,92 @@ __device__ __forceinline__ bool in_binade(long long T, bool neg) {
   return m >= (1ll << 23) + 1 && m <= (1ll << 24) - 1;
 }
 
+// Near zero a voxel's running sum is stepped in fixed point.  With U the
+// smaller ulp of the two increments, every sum the sequence reaches is a
+// multiple of U: the increments are, a rounded sum is a multiple of its own
+// ulp (at least U once it is as large as the increments), and smaller sums
+// are exact.  So v = V U for an integer V, and one rounded step is an
+// integer add and a rounding of the sum X to the grid 2^(lead(|X|) - 23) of
+// its binade: a mask toward -inf, or ties-to-even to nearest.  An exact
+// cancellation gives -0 rounding toward -inf and +0 to nearest.
+struct FixedSteps {
+  int uexp;   // U = 2^uexp
+  int Ah;     // hit increment in units of U
+  int Am;     // miss increment in units of U
+  int bound;  // fixed point while |V| < bound (|v| < small)
+  bool ok;    // the increments and the bound fit the integer range
+};
+
+__device__ __forceinline__ FixedSteps fixed_steps(float hit, float miss,
+                                                  float small) {
+  FixedSteps f{};
+  f.ok = false;
+  const uint32_t eh = (bits_of(hit) >> 23) & 0xffu;
+  const uint32_t em = (bits_of(miss) >> 23) & 0xffu;
+  if (eh == 0u || em == 0u || eh == 0xffu || em == 0xffu) return f;
+  const int e = (int)(eh < em ? eh : em) - 150;
+  if (e < -126) return f;
+  const double inv = double_of((uint64_t)(1023 - e) << 52);  // 2^-e
+  const double ah = (double)hit * inv, am = (double)miss * inv;
+  const double b = (double)small * inv;  // all three exact
+  if (!(abs_(ah) < 134217728.0 && abs_(am) < 134217728.0 && b < 536870912.0))
+    return f;
+  f.uexp = e;
+  f.Ah = (int)ah;
+  f.Am = (int)am;
+  f.bound = (int)-floor_(-b);  // ceil
+  f.ok = true;
+  return f;
+}
+
+// One rounded fixed-point step: X = V + A rounded to its binade's grid;
+// *z takes the signed zero of an exact cancellation.
+template <int M>
+__device__ __forceinline__ int fixed_step(int V, int A, float* z) {
+  const int X = V + A;
+  if (X == 0) {
+    *z = M == kDown ? -0.0f : 0.0f;
+    return 0;
+  }
+  const uint32_t m = (uint32_t)(X < 0 ? -X : X);
+  const int k = 8 - (int)__builtin_clz(m);  // lead(m) - 23
+  if (k <= 0) return X;
+  const int low = (1 << k) - 1;
+  if (M == kDown) return X & ~low;
+  return (X + (low >> 1) + ((X >> k) & 1)) & ~low;
+}
+
+// Events i0.. of a chunk stepped in fixed point while |v| < small (v must
+// lie on the U grid, else nothing is done).  The event whose misses leave
+// the region is finished in floating point.  Returns the next event.
+template <int M>
+__device__ uint32_t fixed_run(const FixedSteps& F, uint32_t g, uint32_t i0,
+                              uint32_t n, float hit, float miss, float& v,
+                              uint32_t& used) {
+  const double inv_u = double_of((uint64_t)(1023 - F.uexp) << 52);
+  const double Vd = (double)v * inv_u;  // exact
+  if (Vd != floor_(Vd)) return i0;
+  const float U = float_of((uint32_t)(F.uexp + 127) << 23);
+  int V = __builtin_amdgcn_readfirstlane((int)Vd);
+  float z = v;
+  while (i0 < n) {
+    uint32_t gi = (uint32_t)__builtin_amdgcn_readlane((int)g, (int)i0);
+    used += gi;
+    for (; gi > 0 && abs(V) < F.bound; --gi) V = fixed_step<M>(V, F.Am, &z);
+    if (abs(V) >= F.bound) {
+      v = (float)V * U;  // exact: V is on its binade's grid
+      v = fold<M, true>(v, miss, gi);
+      v = add<M, true>(v, hit);
+      return i0 + 1;
+    }
+    V = fixed_step<M>(V, F.Ah, &z);
+    ++i0;
+    if (abs(V) >= F.bound) break;
+  }
+  v = V == 0 ? z : (float)V * U;
+  return i0;
+}
+
 template <int M>
 __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
                                    uint32_t j, float miss, float hit,
@@ -779,6 +954,7 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
   const float small = 8.0f * fmaxf(fabsf(hit), fabsf(miss));
   float v = 0.0f;
   uint32_t used = 0;
+  const FixedSteps F = fixed_steps(hit, miss, small);
   // Inside a binade the running sum is kept as S ulps (binade exponent ex,
   // steps Ah and Am ulps); v is brought up to date when the binade is left.
   bool in = false;
@@ -807,10 +983,10 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     const uint32_t n = j - base < 64u ? j - base : 64u;
     const uint32_t g = grp[c];
     uint64_t nzm = __ballot(g != 0u && lane < n);
-    if (!in && __popcll(nzm) <= 8 && fabsf(v) < small) {
-      // near zero (where the prefix sums below step one event at a time)
-      // and few misses in this chunk: runs of hits between them, one fold
-      // each
+    if (!in && !F.ok && __popcll(nzm) <= 8 && fabsf(v) < small) {
+      // near zero without fixed-point steps (where the prefix sums below
+      // step one event at a time) and few misses in this chunk: runs of
+      // hits between them, one fold each
       uint32_t pos = 0;
       while (nzm) {
         const uint32_t p = (uint32_t)__builtin_ctzll(nzm);
@@ -827,6 +1003,13 @@ __device__ float wave_hit_sequence(const uint32_t* __restrict__ gap,
     }
     uint32_t i0 = 0;
     while (i0 < n) {
+      if (!in && F.ok && fabsf(v) < small) {
+        const uint32_t i1 = fixed_run<M>(F, g, i0, n, hit, miss, v, used);
+        if (i1 != i0) {
+          i0 = i1;
+          continue;
+        }
+      }
       if (!in) {
         ex = (bits_of(v) >> 23) & 0xffu;
         if (ex != 0u && ex != 0xffu && !(fabsf(v) < small)) {
