@@ -3,39 +3,63 @@
 // Used to group hit events by voxel (in ascending pixel order) and to bin
 // points by voxel in the voxel filter.  Digits of D = 8..11 bits, chosen so
 // the pass count is minimal (27 key bits: 3 passes of 9); each pass is
-//   hist:    per-tile digit histogram (LDS atomics), digit-major layout;
+//   hist:    per-tile digit histogram, digit-major layout: lanes holding the
+//            same digit are matched with D wavefront ballots and the first
+//            of them adds their number (hit keys repeat a voxel's cell many
+//            times, and a plain LDS atomic per key serializes on them);
 //   rowscan: one workgroup per digit: exclusive prefix over the tiles and
 //            the digit total (the scatter scans the 2^D totals in LDS);
-//   scatter: stable rank of every key inside its tile, computed with
-//            wavefront ballots on the 8 digit bits (a 64-lane match), a
-//            per-wave digit count in LDS and a running per-digit count
-//            across the tile's strided iterations.
-// Tiles are 2048 keys: 256 threads x 8 strided iterations, so iteration j
-// holds keys [j*256, j*256+256) of the tile in lane order and the input
-// order is (iteration, wave, lane) -- the ranking is therefore stable.
+//   scatter: stable rank of every key inside its tile, from the same
+//            ballot match, a per-wave digit count in LDS and a running
+//            per-digit count across the tile's strided iterations, both
+//            updated by the first lane of each match only.
+// A tile is 256 threads x `items` strided iterations, so iteration j holds
+// keys [j*256, j*256+256) of the tile in lane order and the input order is
+// (iteration, wave, lane) -- the ranking is therefore stable.  Small sorts
+// use 8 iterations (2048 keys) for enough tiles to fill the GPU; large
+// ones up to 64, so the digit-major histogram (2^D words per tile, written
+// and read one word per digit row) stays well below the size of the keys.
 #include "nm_internal.h"
 
 namespace nm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = kRadixTile / kThreads;  // 8
+constexpr int kMinItems = kRadixTile / kThreads;  // 8
+constexpr int kMaxItems = 64;
 constexpr int kWaves = kThreads / 64;
+
+// Lanes of the wavefront holding the same digit as this one (D ballots).
+template <int D>
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+    const bool bit = (d >> b) & 1;
+    const uint64_t bb = __ballot(valid && bit);
+    peers &= bit ? bb : ~bb;
+  }
+  return peers;
+}
 
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t* __restrict__ keys, int n, int shift,
-    uint32_t* __restrict__ hist, int nblocks) {
+    uint32_t* __restrict__ hist, int nblocks, int items) {
   constexpr int kBins = 1 << D;
   __shared__ uint32_t h[kBins];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int b = tid; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kRadixTile;
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) {
+  const int base = blockIdx.x * items * kThreads;
+  for (int j = 0; j < items; ++j) {
     const int i = base + j * kThreads + tid;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1)], 1u);
+    const bool valid = i < n;
+    const uint32_t d = valid ? (keys[i] >> shift) & (kBins - 1) : 0u;
+    const uint64_t peers = digit_peers<D>(d, valid);
+    if (valid && (peers & lt) == 0)
+      atomicAdd(&h[d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   for (int b = tid; b < kBins; b += kThreads) hist[b * nblocks + blockIdx.x] = h[b];
@@ -75,7 +99,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
     int shift, const uint32_t* __restrict__ hist, int nblocks,
-    const uint32_t* __restrict__ totals) {
+    const uint32_t* __restrict__ totals, int items) {
   constexpr int kBins = 1 << D;
   __shared__ uint32_t running[kBins];
   __shared__ uint32_t offset[kBins];
@@ -113,14 +137,14 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       pre += loc[q];
     }
   }
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blockIdx.x * kRadixTile;
-  for (int j = 0; j < kItems; ++j) {
-    for (int b = tid; b < kBins; b += kThreads) {
+  for (int b = tid; b < kBins; b += kThreads) {
 #pragma unroll
-      for (int w = 0; w < kWaves; ++w) wcnt[w][b] = 0;
-    }
-    __syncthreads();
+    for (int w = 0; w < kWaves; ++w) wcnt[w][b] = 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int base = blockIdx.x * items * kThreads;
+  for (int j = 0; j < items; ++j) {
     const int i = base + j * kThreads + tid;
     const bool valid = i < n;
     uint32_t key = 0, val = 0, d = 0;
@@ -129,15 +153,11 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       val = vin[i];
       d = (key >> shift) & (kBins - 1);
     }
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < D; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t bb = __ballot(valid && bit);
-      peers &= bit ? bb : ~bb;
-    }
+    const uint64_t peers = digit_peers<D>(d, valid);
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+    const uint32_t cnt = (uint32_t)__popcll(peers);
+    const bool leader = valid && rank == 0;
+    if (leader) wcnt[wave][d] = cnt;
     __syncthreads();
     if (valid) {
       uint32_t pos = offset[d] + running[d] + rank;
@@ -146,24 +166,26 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       vout[pos] = val;
     }
     __syncthreads();
-    for (int b = tid; b < kBins; b += kThreads) {
-      uint32_t tot = 0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) tot += wcnt[w][b];
-      running[b] += tot;
+    // every lane has read: the leaders clear their wave's count and move
+    // the digit's running count on (ordered before the next reads by the
+    // next iteration's barrier)
+    if (leader) {
+      wcnt[wave][d] = 0;
+      atomicAdd(&running[d], cnt);
     }
   }
 }
 
 template <int D>
 void radix_pass(uint32_t* kin, uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                int n, int shift, uint32_t* hist, int nblocks,
+                int n, int shift, uint32_t* hist, int nblocks, int items,
                 hipStream_t s) {
   uint32_t* totals = hist + (size_t)(1 << D) * nblocks;
-  k_radix_hist<D><<<nblocks, kThreads, 0, s>>>(kin, n, shift, hist, nblocks);
+  k_radix_hist<D><<<nblocks, kThreads, 0, s>>>(kin, n, shift, hist, nblocks,
+                                               items);
   k_radix_rowscan<<<1 << D, 256, 0, s>>>(hist, nblocks, totals);
-  k_radix_scatter<D><<<nblocks, kThreads, 0, s>>>(kin, vin, kout, vout, n,
-                                                  shift, hist, nblocks, totals);
+  k_radix_scatter<D><<<nblocks, kThreads, 0, s>>>(
+      kin, vin, kout, vout, n, shift, hist, nblocks, totals, items);
 }
 
 }  // namespace
@@ -177,17 +199,21 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
   int cur = 0;
   *out_buf = 0;
   if (n <= 0) return hipSuccess;
-  const int nblocks = radix_max_blocks(n);
+  // iterations per tile: at least 8, and enough for ~2048 tiles at most
+  int items = kMinItems;
+  while (items < kMaxItems && (int64_t)n > (int64_t)items * kThreads * 2048)
+    items *= 2;
+  const int nblocks = (n + items * kThreads - 1) / (items * kThreads);
   const int passes = (bits + 10) / 11;
   const int D = (bits + passes - 1) / passes < 8 ? 8 : (bits + passes - 1) / passes;
   for (int shift = 0; shift < bits; shift += D) {
     uint32_t *ki = keys[cur], *vi = vals[cur], *ko = keys[cur ^ 1],
              *vo = vals[cur ^ 1];
     switch (D) {
-      case 8: radix_pass<8>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
-      case 9: radix_pass<9>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
-      case 10: radix_pass<10>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
-      default: radix_pass<11>(ki, vi, ko, vo, n, shift, hist, nblocks, s); break;
+      case 8: radix_pass<8>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
+      case 9: radix_pass<9>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
+      case 10: radix_pass<10>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
+      default: radix_pass<11>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
     }
     cur ^= 1;
   }
