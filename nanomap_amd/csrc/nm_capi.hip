@@ -49,6 +49,9 @@ struct nm_ctx {
   hipStream_t stream = nullptr;
   nm_map* last_map = nullptr;
   bool last_update_valid = false;  // slot_stats hold the last frame's sums
+  // the workspace's miss-run table: valid for (miss bits, rounding)
+  bool runs_valid = false;
+  uint64_t runs_key = 0;
   // stage timing
   bool timing = false;
   struct Mark {
@@ -489,7 +492,15 @@ int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   {
     StageTimer t(ctx, NM_STAGE_UPDATE, s);
+    // the table of miss runs is rebuilt only when the miss log-odds or the
+    // rounding mode change
+    const uint64_t key = ((uint64_t)nm::bits_of(ctx->job.miss) << 8) |
+                         (uint64_t)(ctx->pipe.rounding & 0xff);
+    ctx->pipe.miss_runs_ready = ctx->runs_valid && ctx->runs_key == key;
+    ctx->runs_valid = false;
     NM_HIP(nm::launch_update(ctx->pipe, ctx->b, nullptr, map->v, s));
+    ctx->runs_valid = true;
+    ctx->runs_key = key;
   }
   ctx->pending = false;
   ctx->last_update_valid = true;

@@ -1070,12 +1070,24 @@ constexpr uint32_t kHitsMask = kTouchedBit - 1;
 // many hits (a surface close to the sensor, often a whole plane of a leaf)
 // are listed for k_heavy, which gives each its own wavefront.  The result
 // replaces the voxel's miss count in the tile.
+// miss_runs[job][k] = fold(0, miss, k): the sum of a voxel with k misses
+// and no hit.
+template <int M>
+__global__ __launch_bounds__(256) void k_miss_runs(const Job* __restrict__ jobs,
+                                                   float* __restrict__ runs) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const float miss = jobs[blockIdx.y].miss;
+  if (k < kMissRuns)
+    runs[(size_t)blockIdx.y * kMissRuns + k] = fold<M, true>(0.0f, miss, k);
+}
+
 template <int M>
 __global__ __launch_bounds__(256) void k_temp(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ gap, FrameCounters* __restrict__ cnt,
-    uint32_t* __restrict__ heavy_list, int tile_cap, uint32_t heavy_hits) {
+    uint32_t* __restrict__ heavy_list, int tile_cap, uint32_t heavy_hits,
+    const float* __restrict__ miss_runs) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t nvox = (int64_t)min(cnt->n_leaves, tile_cap) * 512;
@@ -1084,7 +1096,8 @@ __global__ __launch_bounds__(256) void k_temp(
        base < nvox; base += stride) {
     // the wavefront's 64 voxels lie in one tile
     const int slot = (int)(base >> 9);
-    const Job& J = jobs[__builtin_amdgcn_readfirstlane(slot_job[slot])];
+    const int job = __builtin_amdgcn_readfirstlane(slot_job[slot]);
+    const Job& J = jobs[job];
     const float hit = J.hit, miss = J.miss;
     const size_t cell = (size_t)base + lane;
     const uint32_t k = tiles[2 * cell];
@@ -1104,7 +1117,8 @@ __global__ __launch_bounds__(256) void k_temp(
     if (!touched || heavy) continue;
     float tv;
     if (j == 0) {
-      tv = fold<M, true>(0.0f, miss, k);
+      tv = k < (uint32_t)kMissRuns ? miss_runs[(size_t)job * kMissRuns + k]
+                                   : fold<M, true>(0.0f, miss, k);
     } else {
       // gaps are loaded 8 at a time so the loads overlap
       uint32_t used = 0;
@@ -1465,6 +1479,13 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
                                    single, p.tile_cap, b.counters, b.map_idx,
                                    b.map_new);
+  const dim3 rgrid(kMissRuns / 256, p.njobs);
+  if (p.miss_runs_ready)
+    ;
+  else if (p.rounding == kDown)
+    k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
+  else
+    k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
   const int tgrid = p.cu_count * 8;
   // one wavefront per heavy voxel and eight per SIMD to hide the latency
   // of its serial chain (the count is on the device: surplus waves exit)
@@ -1472,7 +1493,8 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   if (p.rounding == kDown) {
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                         b.hit_start, b.gap, b.counters,
-                                        b.heavy, p.tile_cap, p.heavy_hits);
+                                        b.heavy, p.tile_cap, p.heavy_hits,
+                                        b.miss_runs);
     k_heavy<kDown><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                          b.hit_start, b.gap, b.counters,
                                          b.heavy);
@@ -1483,7 +1505,8 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   } else {
     k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                            b.hit_start, b.gap, b.counters,
-                                           b.heavy, p.tile_cap, p.heavy_hits);
+                                           b.heavy, p.tile_cap, p.heavy_hits,
+                                        b.miss_runs);
     k_heavy<kNearest><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                             b.hit_start, b.gap, b.counters,
                                             b.heavy);

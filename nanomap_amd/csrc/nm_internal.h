@@ -152,6 +152,7 @@ struct Workspace {
   float* filter_rays;   // [ray_cap * 4]
   float* filter_tmp;    // [ray_cap * 4]
   int32_t* flags;       // [ray_cap]
+  float* miss_runs;     // [max_jobs * kMissRuns]  fold(0, miss, k) per job
 };
 
 // Sizes of a workspace.
@@ -169,19 +170,21 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out);
 void workspace_free(Workspace* b);
 
 constexpr int kScanChunk = 4096;
+// A voxel of a frame that only misses holds fold(0, miss, k): the update
+// reads it from a per-job table for k < kMissRuns.
+constexpr int kMissRuns = 4096;
 // Bit of a cube-table entry marking a leaf that holds hit voxels (set after
 // the hit sort, cleared with the entry by the update).
 constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
 // Voxels with more hits than Pipeline::heavy_hits (at least kMinHeavyHits)
-// in one launch sequence are summed by a whole wavefront each.  A single
-// frame is latency-bound and lists voxels from 16 hits on (no lane sums a
-// long sequence alone); a batch is throughput-bound and lists them from
-// 256 (a lane per voxel uses the machine better).
+// in one launch sequence are summed by a whole wavefront each, so that no
+// lane sums a long sequence alone (measured best from 16 hits on, for a
+// single frame and for a batch).
 constexpr uint32_t kMinHeavyHits = 16;
 constexpr uint32_t kFrameHeavyHits = 16;
-constexpr uint32_t kBatchHeavyHits = 256;
+constexpr uint32_t kBatchHeavyHits = 16;
 
 // ---- launchers (nm_cast.hip, nm_radix.hip, nm_filter.hip) -------------
 // The frame sequence over a device job table `b.jobs[0, njobs)`:
@@ -199,6 +202,7 @@ struct Pipeline {
   int tile_cap;
   int sort_bits;
   uint32_t heavy_hits = kFrameHeavyHits;  // see kMinHeavyHits
+  bool miss_runs_ready = false;  // miss_runs already hold these jobs' table
   int rounding;
   int cu_count;
   Tuning tuning;

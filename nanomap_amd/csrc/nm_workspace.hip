@@ -51,6 +51,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.part_discover, I * 4));
   A(dalloc(&b.item_job, I));
   A(dalloc(&b.item_order, I));
+  A(dalloc(&b.miss_runs, J * kMissRuns));
   if (d.host_clouds) {
     A(dalloc(&b.cloud, P * 16));
     A(dalloc(&b.filter_rays, P * 4));
@@ -83,7 +84,7 @@ void workspace_free(Workspace* b) {
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
                   b->jobs,       b->maps,        b->part_discover,
                   b->item_job,   b->item_order,  b->cloud,      b->filter_rays,
-                  b->filter_tmp, b->flags};
+                  b->filter_tmp, b->flags,       b->miss_runs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   *b = Workspace{};
