@@ -249,6 +249,15 @@ NM_HD double rint_(double x) {
 #endif
 }
 NM_HD double abs_(double x) { return x < 0 ? -x : x; }
+// An approximation of 1/x good to a few ulps (callers correct the integer
+// quotients they derive from it).
+NM_HD double approx_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(x);
+#else
+  return 1.0 / x;
+#endif
+}
 
 //
 // kFinite: v, a and every partial sum are known to be finite (bounded
@@ -278,8 +287,9 @@ NM_HD float fold(float v, float a, uint32_t k) {
     const uint32_t ex = (bits_of(v) >> 23) & 0xff;  // biased exponent >= 1
     const double lo = double_of((uint64_t)(ex + 896) << 52);      // 2^e
     const double ulp = double_of((uint64_t)(ex + 896 - 23) << 52);  // 2^(e-23)
+    const double inv_ulp = double_of((uint64_t)(1173 - ex) << 52);  // 1 / ulp
     const double hi = 2.0 * lo;
-    const double q = (double)a / ulp;  // exact: power-of-two scaling
+    const double q = (double)a * inv_ulp;  // exact: power-of-two scaling
     double dq;
     if (M == kDown) {
       dq = floor_(q);
@@ -302,7 +312,9 @@ NM_HD float fold(float v, float a, uint32_t k) {
       if (X <= 0) {
         n = 0;
       } else {
-        int64_t i = (int64_t)floor_(X / ad);
+        // quotient from an approximate reciprocal, then made exact
+        int64_t i = (int64_t)floor_(X * approx_rcp(ad));
+        if (i < 0) i = 0;
         while (i > 0 && (double)i * ad >= X) --i;
         while ((double)(i + 1) * ad < X) ++i;
         n = ((double)i * ad < X) ? i + 1 : 0;
@@ -313,7 +325,8 @@ NM_HD float fold(float v, float a, uint32_t k) {
       if (Y < 0) {
         n = 0;
       } else {
-        int64_t i = (int64_t)floor_(Y / ad);
+        int64_t i = (int64_t)floor_(Y * approx_rcp(ad));
+        if (i < 0) i = 0;
         while (i > 0 && (double)i * ad > Y) --i;
         while ((double)(i + 1) * ad <= Y) ++i;
         n = i + 1;
