@@ -26,7 +26,7 @@ typedef struct nm_batch nm_batch;
 typedef struct nm_batch_config {
   int device;
   int max_frames;         /* frames per batch                               */
-  int64_t max_points;     /* points of all frames of a batch                */
+  int64_t max_points;     /* points of all frames of a batch, < 2^30        */
   float max_time;         /* largest per-frame max_time (voxels)            */
   int64_t tile_capacity;  /* leaf tiles of all frames of a batch, <= 2^22   */
 } nm_batch_config;

@@ -79,6 +79,10 @@ struct nm_batch {
   Staging st[2];
   int next = 0;
   hipStream_t capture = nullptr;  // graphs are captured on this stream
+  // branch of the update for the heavy voxels (see launch_update)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev = nullptr;
   // last batch
   int last_n = 0;
   std::vector<nm_map*> last_maps;
@@ -118,6 +122,9 @@ void free_batch(nm_batch* b) {
   if (b->host_counters) (void)hipHostFree(b->host_counters);
   if (b->host_stats) (void)hipHostFree(b->host_stats);
   if (b->capture) (void)hipStreamDestroy(b->capture);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
   nm::workspace_free(&b->ws);
   delete b;
 }
@@ -265,6 +272,9 @@ int build_layout(nm_batch* b, const nm_agent_frame* f, int n) {
   p.rounding = b->rounding;
   p.cu_count = b->cu_count;
   p.tuning = b->tuning;
+  p.side = b->side;
+  p.fork = b->fork_ev;
+  p.join = b->join_ev;
   b->layout_valid = true;
   return NM_OK;
 }
@@ -322,7 +332,7 @@ int enqueue(nm_batch* b, Staging& s, hipStream_t q, bool dry) {
   if (dry)
     BH(nm::launch_clear_slots(p, w, q));
   else
-    BH(nm::launch_update(p, w, w.maps, nm::MapView{}, q));
+    BH(nm::launch_update(p, w, sorted, w.maps, nm::MapView{}, q));
   b->sorted_buf = sorted;
   return NM_OK;
 }
@@ -356,7 +366,7 @@ int nm_batch_create(const nm_batch_config* cfg, nm_batch** out) {
   if (!cfg || !out) return bfail(NM_EINVAL, "nm_batch_create: null argument");
   *out = nullptr;
   if (cfg->max_frames <= 0 || cfg->max_points <= 0 ||
-      cfg->max_points > ((int64_t)1 << 31) - 1 || !(cfg->max_time > 0) ||
+      cfg->max_points > ((int64_t)1 << 30) - 1 || !(cfg->max_time > 0) ||
       cfg->max_time > 4000 || cfg->tile_capacity <= 0 ||
       cfg->tile_capacity > (1 << 22))
     return bfail(NM_EINVAL, "nm_batch_create: bad configuration");
@@ -402,6 +412,12 @@ int nm_batch_create(const nm_batch_config* cfg, nm_batch** out) {
                       sizeof(nm::JobStats) * cfg->max_frames);
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     free_batch(b);
     return bfail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,

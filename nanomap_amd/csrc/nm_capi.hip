@@ -498,7 +498,8 @@ int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
                          (uint64_t)(ctx->pipe.rounding & 0xff);
     ctx->pipe.miss_runs_ready = ctx->runs_valid && ctx->runs_key == key;
     ctx->runs_valid = false;
-    NM_HIP(nm::launch_update(ctx->pipe, ctx->b, nullptr, map->v, s));
+    NM_HIP(nm::launch_update(ctx->pipe, ctx->b, ctx->sorted_buf, nullptr,
+                             map->v, s));
     ctx->runs_valid = true;
     ctx->runs_key = key;
   }

@@ -1085,11 +1085,9 @@ template <int M>
 __global__ __launch_bounds__(256) void k_temp(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
-    const uint32_t* __restrict__ gap, FrameCounters* __restrict__ cnt,
-    uint32_t* __restrict__ heavy_list, int tile_cap, uint32_t heavy_hits,
-    const float* __restrict__ miss_runs) {
+    const uint32_t* __restrict__ gap, const FrameCounters* __restrict__ cnt,
+    int tile_cap, uint32_t heavy_hits, const float* __restrict__ miss_runs) {
   const int lane = threadIdx.x & 63;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t nvox = (int64_t)min(cnt->n_leaves, tile_cap) * 512;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
@@ -1105,16 +1103,11 @@ __global__ __launch_bounds__(256) void k_temp(
     const uint32_t s0 = end ? hit_start[cell] : 0u;
     const uint32_t j = end - s0;
     const bool touched = (k | j) != 0;
-    const bool heavy = j > heavy_hits;
-    const uint64_t hv = __ballot(heavy);
-    if (hv) {
-      int first = 0;
-      if (lane == (int)__builtin_ctzll(hv))
-        first = atomicAdd(&cnt->n_heavy, (int)__popcll(hv));
-      first = __shfl(first, (int)__builtin_ctzll(hv), 64);
-      if (heavy) heavy_list[first + (int)__popcll(hv & lt)] = (uint32_t)cell;
-    }
-    if (!touched || heavy) continue;
+    // voxels with more than heavy_hits hits are k_heavy's.  k_heavy may
+    // run alongside: a hit word it has already
+    // rewritten carries kTouchedBit (a run end is below 2^30 points), and
+    // such a voxel is skipped too.
+    if (!touched || j > heavy_hits || end >= kTouchedBit) continue;
     float tv;
     if (j == 0) {
       tv = k < (uint32_t)kMissRuns ? miss_runs[(size_t)job * kMissRuns + k]
@@ -1153,25 +1146,40 @@ __global__ __launch_bounds__(256) void k_heavy(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ gap, const FrameCounters* __restrict__ cnt,
-    const uint32_t* __restrict__ heavy_list) {
+    const uint32_t* __restrict__ keys, uint32_t heavy_hits) {
   const int lane = threadIdx.x & 63;
-  const int nh = cnt->n_heavy;
+  const int nh = cnt->n_hit_rays;
+  const int nchunks = (nh + 63) >> 6;
   const int waves = gridDim.x * (blockDim.x >> 6);
-  for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nh;
-       q += waves) {
-    const size_t cell = heavy_list[q];
-    const int slot = (int)(cell >> 9);
-    const Job& J = jobs[__builtin_amdgcn_readfirstlane(slot_job[slot])];
-    const uint32_t k = tiles[2 * cell];
-    const uint32_t s0 = hit_start[cell];
-    const uint32_t j = tiles[2 * cell + 1] - s0;
-    uint32_t used = 0;
-    const float acc =
-        wave_hit_sequence<M>(gap + s0, j, J.miss, J.hit, &used);
-    const float tv = fold<M, true>(acc, J.miss, k >= used ? k - used : 0u);
-    if (lane == 0) {
-      tiles[2 * cell] = bits_of(tv);
-      tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+  // Each wavefront scans chunks of 64 sorted hits for the first hit of a
+  // voxel whose run is longer than heavy_hits (the run reaches
+  // e + heavy_hits), and sums each such voxel's sequence in turn.
+  for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       c < nchunks; c += waves) {
+    const int e = c * 64 + lane;
+    const uint32_t key = e < nh ? keys[e] : kInvalid;
+    const uint32_t prev = (e > 0 && e <= nh) ? keys[e - 1] : kInvalid;
+    const bool heavy = key != kInvalid && key != prev &&
+                       (int64_t)e + heavy_hits < (int64_t)nh &&
+                       keys[e + heavy_hits] == key;
+    uint64_t m = __ballot(heavy);
+    while (m) {
+      const int l = (int)__builtin_ctzll(m);
+      m &= m - 1;
+      const size_t cell = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
+      const int slot = (int)(cell >> 9);
+      const Job& J = jobs[__builtin_amdgcn_readfirstlane(slot_job[slot])];
+      const uint32_t k = tiles[2 * cell];
+      const uint32_t s0 = hit_start[cell];
+      const uint32_t j = tiles[2 * cell + 1] - s0;
+      uint32_t used = 0;
+      const float acc =
+          wave_hit_sequence<M>(gap + s0, j, J.miss, J.hit, &used);
+      const float tv = fold<M, true>(acc, J.miss, k >= used ? k - used : 0u);
+      if (lane == 0) {
+        tiles[2 * cell] = bits_of(tv);
+        tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+      }
     }
   }
 }
@@ -1474,47 +1482,69 @@ hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
 }
 
 hipError_t launch_update(const Pipeline& p, const Workspace& b,
-                         const MapView* map_table, const MapView& single,
-                         hipStream_t s) {
+                         int sorted_buf, const MapView* map_table,
+                         const MapView& single, hipStream_t s) {
+  const bool down = p.rounding == kDown;
+  // eight wavefronts per SIMD to hide the latency of the heavy voxels'
+  // serial chains
+  const int hgrid = p.cu_count * 8;
+  const int tgrid = p.cu_count * 8;
+  // With a side stream (a branch of the batch graph) the heavy voxels'
+  // chains, which need only the cast's results, run beside the map lookup
+  // and the per-voxel sums; both join before the apply pass.
+  hipStream_t hs = s;
+  if (p.side) {
+    hipError_t e = hipEventRecord(p.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p.side, p.fork, 0);
+    if (e != hipSuccess) return e;
+    hs = p.side;
+  }
+  if (down)
+    k_heavy<kDown><<<hgrid, 256, 0, hs>>>(b.jobs, b.slot_job, b.tiles,
+                                          b.hit_start, b.gap, b.counters,
+                                          b.keys[sorted_buf], p.heavy_hits);
+  else
+    k_heavy<kNearest><<<hgrid, 256, 0, hs>>>(b.jobs, b.slot_job, b.tiles,
+                                             b.hit_start, b.gap, b.counters,
+                                             b.keys[sorted_buf],
+                                             p.heavy_hits);
+  if (p.side) {
+    const hipError_t e = hipEventRecord(p.join, p.side);
+    if (e != hipSuccess) return e;
+  }
   k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
                                    single, p.tile_cap, b.counters, b.map_idx,
                                    b.map_new);
   const dim3 rgrid(kMissRuns / 256, p.njobs);
-  if (p.miss_runs_ready)
-    ;
-  else if (p.rounding == kDown)
-    k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
-  else
-    k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
-  const int tgrid = p.cu_count * 8;
-  // one wavefront per heavy voxel and eight per SIMD to hide the latency
-  // of its serial chain (the count is on the device: surplus waves exit)
-  const int hgrid = p.cu_count * 8;
-  if (p.rounding == kDown) {
+  if (!p.miss_runs_ready) {
+    if (down)
+      k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
+    else
+      k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
+  }
+  if (down)
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
                                         b.hit_start, b.gap, b.counters,
-                                        b.heavy, p.tile_cap, p.heavy_hits,
-                                        b.miss_runs);
-    k_heavy<kDown><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
-                                         b.hit_start, b.gap, b.counters,
-                                         b.heavy);
+                                        p.tile_cap, p.heavy_hits, b.miss_runs);
+  else
+    k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
+                                           b.hit_start, b.gap, b.counters,
+                                           p.tile_cap, p.heavy_hits,
+                                           b.miss_runs);
+  if (p.side) {
+    const hipError_t e = hipStreamWaitEvent(s, p.join, 0);
+    if (e != hipSuccess) return e;
+  }
+  if (down)
     k_apply<kDown><<<kUpdateGrid, 512, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
         b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
         b.slot_stats);
-  } else {
-    k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
-                                           b.hit_start, b.gap, b.counters,
-                                           b.heavy, p.tile_cap, p.heavy_hits,
-                                        b.miss_runs);
-    k_heavy<kNearest><<<hgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
-                                            b.hit_start, b.gap, b.counters,
-                                            b.heavy);
+  else
     k_apply<kNearest><<<kUpdateGrid, 512, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
         b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
         b.slot_stats);
-  }
   return hipGetLastError();
 }
 

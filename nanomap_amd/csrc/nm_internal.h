@@ -30,7 +30,6 @@ struct FrameCounters {
   int32_t n_filter_rays;    // rays emitted by the voxel filter
   int32_t n_filter_points;  // points binned by the voxel filter
   int32_t queue[4];         // work-queue heads of the persistent kernels
-  int32_t n_heavy;          // voxels with many hits, listed for k_heavy
 };
 
 // Statistics of one job, reduced on demand from per-item and per-slot
@@ -136,8 +135,6 @@ struct Workspace {
   uint32_t* keys[2];    // [ray_cap]  radix sort buffers
   uint32_t* vals[2];
   uint32_t* gap;        // [ray_cap]  misses before each sorted hit
-  uint32_t* heavy;      // [ray_cap / kMinHeavyHits + 1]  cells of voxels
-                        //             with more than heavy_hits hits
   uint32_t* radix_hist; // radix_hist_words(ray_cap)
   int32_t* scan_blocks; // [cube_cap / kScanChunk + 2]
   int32_t* scan_job;    // [cube_cap / kScanChunk + 2]  job of each chunk
@@ -178,11 +175,10 @@ constexpr int kMissRuns = 4096;
 constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
 constexpr int kRadixTile = 2048;
-// Voxels with more hits than Pipeline::heavy_hits (at least kMinHeavyHits)
-// in one launch sequence are summed by a whole wavefront each, so that no
-// lane sums a long sequence alone (measured best from 16 hits on, for a
-// single frame and for a batch).
-constexpr uint32_t kMinHeavyHits = 16;
+// Voxels with more hits than Pipeline::heavy_hits in one launch sequence
+// are summed by a whole wavefront each, so that no lane sums a long
+// sequence alone (measured best from 16 hits on, for a single frame and for
+// a batch).
 constexpr uint32_t kFrameHeavyHits = 16;
 constexpr uint32_t kBatchHeavyHits = 16;
 
@@ -201,8 +197,12 @@ struct Pipeline {
   int nitems;         // cast work items of all jobs
   int tile_cap;
   int sort_bits;
-  uint32_t heavy_hits = kFrameHeavyHits;  // see kMinHeavyHits
+  uint32_t heavy_hits = kFrameHeavyHits;  // see kFrameHeavyHits
   bool miss_runs_ready = false;  // miss_runs already hold these jobs' table
+  // optional second stream for the heavy voxels (fork/join events)
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr;
+  hipEvent_t join = nullptr;
   int rounding;
   int cu_count;
   Tuning tuning;
@@ -221,8 +221,8 @@ hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
                              int sorted_buf, hipStream_t s);
 // map_table: device table indexed by Job::map, or null to use `single`.
 hipError_t launch_update(const Pipeline& p, const Workspace& b,
-                         const MapView* map_table, const MapView& single,
-                         hipStream_t s);
+                         int sorted_buf, const MapView* map_table,
+                         const MapView& single, hipStream_t s);
 // Clears the per-frame state of the last cast without touching a map.
 hipError_t launch_clear_slots(const Pipeline& p, const Workspace& b,
                               hipStream_t s);

@@ -40,7 +40,6 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.vals[0], P));
   A(dalloc(&b.vals[1], P));
   A(dalloc(&b.gap, P));
-  A(dalloc(&b.heavy, P / kMinHeavyHits + 1));
   A(dalloc(&b.radix_hist, (size_t)radix_hist_words((int)P)));
   A(dalloc(&b.scan_blocks, scan_blocks));
   A(dalloc(&b.scan_job, scan_blocks));
@@ -79,7 +78,7 @@ void workspace_free(Workspace* b) {
   void* ptrs[] = {b->leaf_slot,  b->slot_leaf,   b->slot_job,   b->map_idx,
                   b->map_new,    b->slot_stats,  b->tiles,      b->hit_mask,
                   b->hit_start,  b->ray_hit,     b->keys[0],    b->keys[1],
-                  b->vals[0],    b->vals[1],     b->gap,        b->heavy,
+                  b->vals[0],    b->vals[1],     b->gap,
                   b->radix_hist,
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
                   b->jobs,       b->maps,        b->part_discover,
