@@ -49,6 +49,10 @@ struct nm_ctx {
   hipStream_t stream = nullptr;
   nm_map* last_map = nullptr;
   bool last_update_valid = false;  // slot_stats hold the last frame's sums
+  // branch of the update for the heavy voxels (see launch_update)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev = nullptr;
   // the workspace's miss-run table: valid for (miss bits, rounding)
   bool runs_valid = false;
   uint64_t runs_key = 0;
@@ -146,6 +150,9 @@ void free_ctx(nm_ctx* c) {
     (void)hipEventDestroy(m.b);
   }
   for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
   nm::workspace_free(&c->b);
   if (c->host_counters) (void)hipHostFree(c->host_counters);
   if (c->host_stats) (void)hipHostFree(c->host_stats);
@@ -208,6 +215,11 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
     e = hipHostMalloc((void**)&c->host_counters, sizeof(nm::FrameCounters));
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&c->host_stats, sizeof(nm::JobStats));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     free_ctx(c);
     return fail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
@@ -443,6 +455,9 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   pl.heavy_hits = nm::kFrameHeavyHits;
   pl.rounding = frame->rounding;
   pl.cu_count = ctx->cu_count;
+  pl.side = ctx->side;
+  pl.fork = ctx->fork_ev;
+  pl.join = ctx->join_ev;
   ctx->job = p;
 
   NM_HIP(nm::launch_begin(ctx->b, &p, s));
