@@ -192,3 +192,31 @@ def test_batch_rejects_bad_input(cfg4):
         batch.submit([(f0, clouds[0], m), (f2, clouds[1], m2)])
     with pytest.raises(nm.NanoMapError):  # stats before any batch
         batch.stats()
+
+
+def test_batch_with_per_agent_log_odds(cfg4):
+    """Agents with different hit/miss log-odds in one batch: each job's
+    table of miss-only sums and its heavy-voxel sums use its own values."""
+    nm, a0, _ = cfg4
+    agents = _subset(a0, 10)
+    clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
+    P = max(c.shape[0] for c in clouds)
+    T = max(a["range"] / a["res"] for a in agents) + 1
+    odds = [(0.85, -0.4), (0.7, -0.3), (1.2, -0.55), (0.4, -0.9)]
+
+    def frame(k, a):
+        f = W.agent_frame(nm, a, RD)
+        f.logodds_hit, f.logodds_miss = odds[k % len(odds)]
+        return f
+
+    ctx = nm.Context(P, T)
+    pmaps = _maps(nm, agents)
+    for k, (a, c, m) in enumerate(zip(agents, clouds, pmaps)):
+        ctx.submit(m, frame(k, a), c)
+    batch = nm.Batch(len(agents), sum(c.shape[0] for c in clouds), T, 1 << 19)
+    bmaps = _maps(nm, agents)
+    batch.submit([(frame(k, a), c, m)
+                  for k, (a, c, m) in enumerate(zip(agents, clouds, bmaps))])
+    torch.cuda.synchronize()
+    for k, (bm, pm) in enumerate(zip(bmaps, pmaps)):
+        _assert_same_map(bm, pm, k)

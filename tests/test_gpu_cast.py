@@ -308,3 +308,22 @@ def test_many_hit_voxels_bit_exact(gpu, rounding):
               flush=True)
         assert (hh > 256).sum() >= 10 and ((hh > 256) & (m > 0)).sum() >= 3
     _check_maps(gmap, omap)
+
+
+def test_log_odds_and_rounding_change_between_frames(gpu):
+    """One context and map over frames whose log-odds and rounding mode
+    change from frame to frame: the context's table of miss-only sums must
+    follow the frame (it is rebuilt only when they change)."""
+    nm = gpu
+    _, cam, frames = W.cfg2_frames(nm, n_frames=5)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    settings = [(0.85, -0.4, RD), (0.7, -0.3, RD), (0.7, -0.3, RN),
+                (0.7, -0.3, RN), (0.85, -0.4, RD)]
+    for (pos, quat, cloud), (hit, miss, rounding) in zip(frames, settings):
+        f = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 10.0, hit, miss,
+                               rounding, cam.width, cam.height)
+        _run_frame(nm, ctx, gmap, omap, f, cloud)
+        assert ctx.stats()["errors"] == 0
+        _check_maps(gmap, omap)
