@@ -42,9 +42,14 @@ def parse():
                     help="timed passes of the 32-agent batch (0 = skip)")
     ap.add_argument("--soft-steps", type=int, default=10,
                     help="timed passes of the soft march (0 = skip)")
-    ap.add_argument("--only", choices=["all", "batch", "soft"], default="all",
+    ap.add_argument("--sim-steps", type=int, default=5,
+                    help="timed passes of the 64-pose sensor simulation "
+                         "(0 = skip)")
+    ap.add_argument("--only", choices=["all", "batch", "soft", "sim"],
+                    default="all",
                     help="run one part only (for profiling): the 32-agent "
-                         "batch or the soft march; prints that part's JSON")
+                         "batch, the soft march or the sensor simulation; "
+                         "prints that part's JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed steps, run one fill and one "
                          "elementwise copy of this many MiB (the known-size "
@@ -141,6 +146,42 @@ def bench_batch(nm, W, torch, steps, rounding):
         "speedup": plain_ms / batch_ms,
         "tuning": tuning,
         "tuner_s": tune_s,
+        "steps": steps,
+    }
+
+
+def bench_sim(nm, W, torch, steps):
+    """Sensor-simulation cast (generateCloud): 64 poses of 848x480 at 90x60
+    deg, 15 m range, through the 0.1 m occupancy grid of the config-1 map
+    thresholded at log-odds > 0; each pass casts all 64 poses in one call
+    (which returns the hit count, so it synchronises)."""
+    o, v = W.cfg1_map_leaves(nm)
+    grid = nm.SimGrid(o, v, 0.0)
+    w, h, res = 848, 480, 0.1
+    sensors = []
+    for pos, quat in W.cfg3_poses(nm, 64):
+        fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, res, 15.0, 0.85, -0.4,
+                                0)
+        sensors.append(nm.sim_sensor(pos, list(fr.rotation), res, w, h, 90.0,
+                                     60.0, 0.1, 15.0))
+    n = 0
+    for _ in range(2):
+        _, _, _, n = grid.cast(sensors)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, _, _, n = grid.cast(sensors)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    rays = len(sensors) * w * h
+    return {
+        "workload": "64 poses x 848x480 (90x60 deg, 15 m) through the 0.1 m "
+                    "grid of the config-1 map, log-odds > 0",
+        "rays": rays,
+        "returns": n,
+        "ms_per_pass": ms,
+        "mrays_per_s": rays / (ms * 1e-3) / 1e6,
+        "active_voxels": grid.info()["active"],
         "steps": steps,
     }
 
@@ -309,6 +350,10 @@ def main():
         print(json.dumps({"soft_march": bench_soft(nm, W, torch,
                                                    max(args.soft_steps, 1))}))
         return
+    if args.only == "sim":
+        print(json.dumps({"sim_cast": bench_sim(nm, W, torch,
+                                                max(args.sim_steps, 1))}))
+        return
     t_setup = time.perf_counter()
     scene, cam, frames = W.cfg2_frames(nm, n_frames=args.frames)
     P = cam.width * cam.height
@@ -350,11 +395,24 @@ def main():
     torch.cuda.synchronize()
     print("warm", file=sys.stderr, flush=True)
 
+    # stage split from an untimed pass with an event pair around every stage
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    for k in range(min(args.steps, 20)):
+        submit(k % args.frames)
+    torch.cuda.synchronize()
+    ctx.set_timing(False)
+    stages = ctx.stage_times()
+    split = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
+    dom = max(split, key=split.get)
+
     K = args.steps
     ev = [(torch.cuda.Event(enable_timing=True),
            torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    # HIP events on the submit stream around the dominant stage only, over
+    # the timed steps, give its launch duration for the roofline
     ctx.reset_timing()
-    ctx.set_timing(True)
+    ctx.set_timing(True, stages=[dom])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
@@ -384,9 +442,41 @@ def main():
         torch.cuda.synchronize()
         pl.append(a.elapsed_time(b))
 
-    # roofline of the dominant kernel stage
-    avg = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
-    dom = max(avg, key=avg.get)
+    # the same frames through the voxel filter (filterCloud precision 0):
+    # points binned per voxel, one ray per occupied voxel, into a second map
+    nf = min(20, args.frames)
+    fparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width, cam.height,
+                       nm.FILTER_VOXEL) for p, q, _ in frames[:nf]]
+    fmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
+    for k in range(2):
+        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    fa, fb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fa.record(stream)
+    for k in range(nf):
+        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+    fb.record(stream)
+    torch.cuda.synchronize()
+    ctx.set_timing(False)
+    fst = ctx.stage_times()
+    filtered = {
+        "workload": "the first %d frames of the sequence with the voxel "
+                    "filter: one ray per occupied 0.05 m voxel" % nf,
+        "frames": nf,
+        "ms_per_frame": fa.elapsed_time(fb) / nf,
+        "rays_per_frame": ctx.stats()["rays"],
+        "stage_ms": {k: round(v / n, 5) if n else 0.0
+                     for k, (v, n) in fst.items()},
+    }
+    fmap.close()
+
+    # roofline of the dominant kernel stage: its duration over the timed
+    # steps; the other stages from the untimed split
+    avg = dict(split)
+    ms_dom, n_dom = stages[dom]
+    avg[dom] = ms_dom / n_dom if n_dom else split[dom]
     Pf = float(P)
     U = voxels / K
     L = leaves / K
@@ -456,6 +546,7 @@ def main():
             "frac": frame_bytes / (frame_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": None,
+        "voxel_filter": filtered,
     }
     del dev, ctx, gmap
     if args.batch_steps > 0:
@@ -464,6 +555,9 @@ def main():
     if args.soft_steps > 0:
         out["soft_march"] = bench_soft(nm, W, torch, args.soft_steps)
         print("soft march done", file=sys.stderr, flush=True)
+    if args.sim_steps > 0:
+        out["sim_cast"] = bench_sim(nm, W, torch, args.sim_steps)
+        print("sim cast done", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(params, clouds,
                                            min(args.cpu_frames, args.frames))

@@ -175,6 +175,10 @@ int nm_filter_rays_download(nm_ctx* ctx, int max_rays, int* n_rays,
 #define NM_STAGE_UPDATE 5     /* temp values, clamped log-odds map update  */
 #define NM_STAGE_COUNT 6
 int nm_ctx_set_timing(nm_ctx* ctx, int enable);
+/* Times only the stages whose bit (1 << NM_STAGE_*) is set; 0 turns timing
+ * off.  Fewer event pairs disturb the timed work less.  No counterpart in
+ * the reference (which prints host timers, Manager.h:65). */
+int nm_ctx_set_timing_mask(nm_ctx* ctx, unsigned mask);
 int nm_ctx_stage_times(nm_ctx* ctx, float* ms, int64_t* launches);
 int nm_ctx_reset_timing(nm_ctx* ctx);
 

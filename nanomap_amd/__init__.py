@@ -127,8 +127,16 @@ class Context:
     STAGES = ("filter", "discover", "compact", "hitsort", "accumulate",
               "update")
 
-    def set_timing(self, enable):
-        check(lib().nm_ctx_set_timing(self.handle, int(bool(enable))))
+    def set_timing(self, enable, stages=None):
+        """Per-stage HIP event timing on or off; `stages` (names from
+        STAGES) limits it to those stages."""
+        if stages is None or not enable:
+            check(lib().nm_ctx_set_timing(self.handle, int(bool(enable))))
+        else:
+            mask = 0
+            for st in stages:
+                mask |= 1 << self.STAGES.index(st)
+            check(lib().nm_ctx_set_timing_mask(self.handle, mask))
 
     def reset_timing(self):
         check(lib().nm_ctx_reset_timing(self.handle))

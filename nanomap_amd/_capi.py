@@ -145,6 +145,7 @@ SIGNATURES = {
     "nm_filter_rays_download": (_I, [_P, _I, ctypes.POINTER(_I), _PF]),
     "nm_fold_host": (_F, [_F, _F, ctypes.c_uint32, _I]),
     "nm_ctx_set_timing": (_I, [_P, _I]),
+    "nm_ctx_set_timing_mask": (_I, [_P, ctypes.c_uint]),
     "nm_ctx_stage_times": (_I, [_P, _PF, _PI64]),
     "nm_ctx_reset_timing": (_I, [_P]),
     "nm_batch_create": (_I, [ctypes.POINTER(BatchConfig),

@@ -58,6 +58,7 @@ struct nm_ctx {
   uint64_t runs_key = 0;
   // stage timing
   bool timing = false;
+  unsigned timing_mask = ~0u;  // stages timed while timing is on
   struct Mark {
     int stage;
     hipEvent_t a, b;
@@ -116,7 +117,7 @@ struct StageTimer {
   hipStream_t s;
   hipEvent_t a = nullptr;
   StageTimer(nm_ctx* c_, int stage_, hipStream_t s_) : c(c_), stage(stage_), s(s_) {
-    if (!c->timing) return;
+    if (!c->timing || !((c->timing_mask >> stage) & 1u)) return;
     a = take_event(c);
     if (a) (void)hipEventRecord(a, s);
   }
@@ -639,6 +640,14 @@ int nm_filter_rays_download(nm_ctx* ctx, int max_rays, int* n_rays,
 int nm_ctx_set_timing(nm_ctx* ctx, int enable) {
   if (!ctx) return fail(NM_EINVAL, "nm_ctx_set_timing: null context");
   ctx->timing = enable != 0;
+  ctx->timing_mask = ~0u;
+  return NM_OK;
+}
+
+int nm_ctx_set_timing_mask(nm_ctx* ctx, unsigned mask) {
+  if (!ctx) return fail(NM_EINVAL, "nm_ctx_set_timing_mask: null context");
+  ctx->timing = mask != 0;
+  ctx->timing_mask = mask;
   return NM_OK;
 }
 
