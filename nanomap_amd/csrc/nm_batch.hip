@@ -326,7 +326,7 @@ int enqueue(nm_batch* b, Staging& s, hipStream_t q, bool dry) {
   BH(nm::launch_hit_keys(p, w, q));
   int sorted = 0;
   BH(nm::radix_sort_pairs(w.keys, w.vals, (int)p.n_rays, p.sort_bits,
-                          w.radix_hist, &sorted, q));
+                          w.radix_hist, &sorted, q, &w.counters->n_hit_keys));
   BH(nm::launch_hit_segments(p, w, sorted, q));
   BH(nm::launch_accumulate(p, w, sorted, q));
   if (dry)

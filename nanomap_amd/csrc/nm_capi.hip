@@ -482,7 +482,8 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     StageTimer t(ctx, NM_STAGE_HITSORT, s);
     NM_HIP(nm::launch_hit_keys(pl, ctx->b, s));
     NM_HIP(nm::radix_sort_pairs(ctx->b.keys, ctx->b.vals, n_points,
-                                pl.sort_bits, ctx->b.radix_hist, &sorted, s));
+                                pl.sort_bits, ctx->b.radix_hist, &sorted, s,
+                                &ctx->b.counters->n_hit_keys));
     NM_HIP(nm::launch_hit_segments(pl, ctx->b, sorted, s));
   }
   {

@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_hit_segments(
     uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
     uint32_t* __restrict__ hit_start, uint32_t* __restrict__ gap,
     const int32_t* __restrict__ slot_leaf, int32_t* __restrict__ leaf_slot) {
-  const int nh = cnt->n_hit_rays;
+  const int nh = cnt->n_hit_keys;
   for (int e = blockIdx.x * 256 + threadIdx.x; e < nh;
        e += gridDim.x * 256) {
     gap[e] = 0;
@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(256) void k_heavy(
     const uint32_t* __restrict__ gap, const FrameCounters* __restrict__ cnt,
     const uint32_t* __restrict__ keys, uint32_t heavy_hits) {
   const int lane = threadIdx.x & 63;
-  const int nh = cnt->n_hit_rays;
+  const int nh = cnt->n_hit_keys;
   const int nchunks = (nh + 63) >> 6;
   const int waves = gridDim.x * (blockDim.x >> 6);
   // Each wavefront scans chunks of 64 sorted hits for the first hit of a

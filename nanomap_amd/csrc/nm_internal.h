@@ -23,6 +23,7 @@ constexpr int kMaskWords = 16;    // 512 bits
 struct FrameCounters {
   int32_t n_leaves;    // leaves discovered (tile slots in use), all jobs
   int32_t n_hit_rays;  // rays whose end voxel received a hit, all jobs
+  int32_t n_hit_keys;  // sorted hit events (hits inside allocated leaves)
   int32_t errors;      // consistency errors (expected 0)
   int32_t overflow;    // tile or map capacity exceeded
   uint64_t rays;       // valid rays
@@ -234,10 +235,12 @@ hipError_t launch_map_clear(const MapView& m, hipStream_t s);
 int job_items(const Job& j, int layout);
 
 // Stable LSD radix sort of (keys, vals) over the low `bits` key bits.
-// Sorts buffer 0 -> result in buffer returned via *out_buf (0 or 1).
+// Sorts buffer 0 -> result in buffer returned via *out_buf (0 or 1).  With
+// n_kept (a device address), keys equal to kInvalid are dropped and the
+// number of keys kept, which lead the result, is stored there.
 hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
                             int bits, uint32_t* hist, int* out_buf,
-                            hipStream_t s);
+                            hipStream_t s, int* n_kept = nullptr);
 int radix_max_blocks(int n);
 int radix_hist_words(int n);  // histogram buffer size for n keys
 

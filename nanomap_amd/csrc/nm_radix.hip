@@ -13,6 +13,10 @@
 //            ballot match, a per-wave digit count in LDS and a running
 //            per-digit count across the tile's strided iterations, both
 //            updated by the first lane of each match only.
+// Optionally the first pass drops keys equal to kInvalid (the hit sort's
+// rays without a hit, a quarter to a third of all points): they are not
+// counted or written, the first pass's scatter stores the number kept, and
+// every later pass reads its key count from there.
 // A tile is 256 threads x `items` strided iterations, so iteration j holds
 // keys [j*256, j*256+256) of the tile in lane order and the input order is
 // (iteration, wave, lane) -- the ranking is therefore stable.  Small sorts
@@ -45,18 +49,21 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t* __restrict__ keys, int n, int shift,
-    uint32_t* __restrict__ hist, int nblocks, int items) {
+    uint32_t* __restrict__ hist, int nblocks, int items,
+    const int* __restrict__ n_dev, bool drop) {
   constexpr int kBins = 1 << D;
   __shared__ uint32_t h[kBins];
   const int tid = threadIdx.x, lane = tid & 63;
+  if (n_dev) n = *n_dev;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int b = tid; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
   const int base = blockIdx.x * items * kThreads;
   for (int j = 0; j < items; ++j) {
     const int i = base + j * kThreads + tid;
-    const bool valid = i < n;
-    const uint32_t d = valid ? (keys[i] >> shift) & (kBins - 1) : 0u;
+    const uint32_t key = i < n ? keys[i] : kInvalid;
+    const bool valid = i < n && !(drop && key == kInvalid);
+    const uint32_t d = valid ? (key >> shift) & (kBins - 1) : 0u;
     const uint64_t peers = digit_peers<D>(d, valid);
     if (valid && (peers & lt) == 0)
       atomicAdd(&h[d], (uint32_t)__popcll(peers));
@@ -99,7 +106,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
     int shift, const uint32_t* __restrict__ hist, int nblocks,
-    const uint32_t* __restrict__ totals, int items) {
+    const uint32_t* __restrict__ totals, int items,
+    const int* __restrict__ n_dev, bool drop, int* __restrict__ n_out) {
   constexpr int kBins = 1 << D;
   __shared__ uint32_t running[kBins];
   __shared__ uint32_t offset[kBins];
@@ -107,6 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   __shared__ uint32_t ws[kWaves];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  if (n_dev) n = *n_dev;
   // digit base = exclusive scan of the digit totals, in LDS
   {
     constexpr int kPer = kBins / kThreads > 0 ? kBins / kThreads : 1;
@@ -136,6 +145,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       }
       pre += loc[q];
     }
+    // the last thread's running sum is the number of keys kept
+    if (n_out && blockIdx.x == 0 && tid == kThreads - 1) *n_out = (int)pre;
   }
   for (int b = tid; b < kBins; b += kThreads) {
 #pragma unroll
@@ -146,10 +157,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   const int base = blockIdx.x * items * kThreads;
   for (int j = 0; j < items; ++j) {
     const int i = base + j * kThreads + tid;
-    const bool valid = i < n;
-    uint32_t key = 0, val = 0, d = 0;
+    uint32_t key = i < n ? kin[i] : kInvalid;
+    const bool valid = i < n && !(drop && key == kInvalid);
+    uint32_t val = 0, d = 0;
     if (valid) {
-      key = kin[i];
       val = vin[i];
       d = (key >> shift) & (kBins - 1);
     }
@@ -179,13 +190,14 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 template <int D>
 void radix_pass(uint32_t* kin, uint32_t* vin, uint32_t* kout, uint32_t* vout,
                 int n, int shift, uint32_t* hist, int nblocks, int items,
-                hipStream_t s) {
+                const int* n_dev, bool drop, int* n_out, hipStream_t s) {
   uint32_t* totals = hist + (size_t)(1 << D) * nblocks;
   k_radix_hist<D><<<nblocks, kThreads, 0, s>>>(kin, n, shift, hist, nblocks,
-                                               items);
+                                               items, n_dev, drop);
   k_radix_rowscan<<<1 << D, 256, 0, s>>>(hist, nblocks, totals);
   k_radix_scatter<D><<<nblocks, kThreads, 0, s>>>(
-      kin, vin, kout, vout, n, shift, hist, nblocks, totals, items);
+      kin, vin, kout, vout, n, shift, hist, nblocks, totals, items, n_dev,
+      drop, n_out);
 }
 
 }  // namespace
@@ -195,7 +207,7 @@ int radix_hist_words(int n) { return 2048 * (radix_max_blocks(n) + 2); }
 
 hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
                             int bits, uint32_t* hist, int* out_buf,
-                            hipStream_t s) {
+                            hipStream_t s, int* n_kept) {
   int cur = 0;
   *out_buf = 0;
   if (n <= 0) return hipSuccess;
@@ -209,11 +221,17 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
   for (int shift = 0; shift < bits; shift += D) {
     uint32_t *ki = keys[cur], *vi = vals[cur], *ko = keys[cur ^ 1],
              *vo = vals[cur ^ 1];
+    // with n_kept: the first pass drops kInvalid keys and stores the count
+    // kept, which the later passes read
+    const bool first = shift == 0;
+    const int* n_dev = (n_kept && !first) ? n_kept : nullptr;
+    const bool drop = n_kept && first;
+    int* n_out = first ? n_kept : nullptr;
     switch (D) {
-      case 8: radix_pass<8>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
-      case 9: radix_pass<9>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
-      case 10: radix_pass<10>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
-      default: radix_pass<11>(ki, vi, ko, vo, n, shift, hist, nblocks, items, s); break;
+      case 8: radix_pass<8>(ki, vi, ko, vo, n, shift, hist, nblocks, items, n_dev, drop, n_out, s); break;
+      case 9: radix_pass<9>(ki, vi, ko, vo, n, shift, hist, nblocks, items, n_dev, drop, n_out, s); break;
+      case 10: radix_pass<10>(ki, vi, ko, vo, n, shift, hist, nblocks, items, n_dev, drop, n_out, s); break;
+      default: radix_pass<11>(ki, vi, ko, vo, n, shift, hist, nblocks, items, n_dev, drop, n_out, s); break;
     }
     cur ^= 1;
   }
