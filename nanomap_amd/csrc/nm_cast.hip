@@ -483,7 +483,9 @@ __global__ __launch_bounds__(256) void k_hit_segments(
 }
 
 // LDS aggregation of miss counts: an open-addressing table of (key,
-// count) per workgroup.  A key is a voxel cell (slot * 512 + local index,
+// count) per workgroup.  Counts are 16-bit halves of 32-bit words (slot h
+// in half h & 1 of word h >> 1): a key gains at most one per ray of the
+// 256-ray item, and the smaller table fits one more workgroup per CU.  A key is a voxel cell (slot * 512 + local index,
 // below 2^31), whose count goes to its miss counter, or kGapKey | i, whose
 // count goes to gap[i] (misses of a hit voxel before its i-th sorted hit).  A 16x16 pixel tile touches a few thousand distinct
 // voxels for tens of thousands of visits, so the table turns almost every
@@ -528,8 +530,8 @@ __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
   // compiler merges them into one flat atomic, which also waits on the
   // global memory counter.
   if (found >= 0)
-    __hip_atomic_fetch_add(&s_cnt[found], inc, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&s_cnt[found >> 1], inc << ((found & 1) * 16),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   else
     atomicAdd(key_counter(tiles, gap, cell), inc);
 }
@@ -588,10 +590,10 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     FrameCounters* __restrict__ cnt) {
   constexpr int kSlots = 1 << S;
   __shared__ uint32_t s_key[kSlots];
-  __shared__ uint32_t s_cnt[kSlots];
+  __shared__ uint32_t s_cnt[kSlots / 2];
   for (int k = threadIdx.x; k < kSlots; k += kCastThreads) {
     s_key[k] = kInvalid;
-    s_cnt[k] = 0;
+    if (k < kSlots / 2) s_cnt[k] = 0;
   }
   int round = 0;
   int errors = 0;
@@ -651,12 +653,21 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     }
     __syncthreads();
     // flush the item's counts and leave the table empty for the next one
-    for (int k = threadIdx.x; k < kSlots; k += kCastThreads) {
-      const uint32_t key = s_key[k];
-      if (key != kInvalid) {
-        atomicAdd(key_counter(tiles, gap, key), s_cnt[k]);
-        s_key[k] = kInvalid;
-        s_cnt[k] = 0;
+    for (int w = threadIdx.x; w < kSlots / 2; w += kCastThreads) {
+      const uint32_t c2 = s_cnt[w];
+      if (c2 != 0u) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const uint32_t key = s_key[2 * w + u];
+          if (key != kInvalid) {
+            atomicAdd(key_counter(tiles, gap, key), (c2 >> (16 * u)) & 0xffffu);
+            s_key[2 * w + u] = kInvalid;
+          }
+        }
+        s_cnt[w] = 0;
+      } else {
+        s_key[2 * w] = kInvalid;
+        s_key[2 * w + 1] = kInvalid;
       }
     }
   }
