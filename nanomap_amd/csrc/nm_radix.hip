@@ -3,10 +3,12 @@
 // Used to group hit events by voxel (in ascending pixel order) and to bin
 // points by voxel in the voxel filter.  Digits of D = 8..11 bits, chosen so
 // the pass count is minimal (27 key bits: 3 passes of 9); each pass is
-//   hist:    per-tile digit histogram, digit-major layout: lanes holding the
-//            same digit are matched with D wavefront ballots and the first
-//            of them adds their number (hit keys repeat a voxel's cell many
-//            times, and a plain LDS atomic per key serializes on them);
+//   hist:    per-tile digit histogram, digit-major layout.  In sorts of
+//            more than 2^20 keys, lanes holding the same digit are matched
+//            with D wavefront ballots and the first of them adds their
+//            number (hit keys repeat a voxel's cell many times, and a plain
+//            LDS atomic per key serializes on them); smaller sorts add one
+//            per key;
 //   rowscan: one workgroup per digit: exclusive prefix over the tiles and
 //            the digit total (the scatter scans the 2^D totals in LDS);
 //   scatter: stable rank of every key inside its tile, from the same
@@ -50,7 +52,7 @@ template <int D>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t* __restrict__ keys, int n, int shift,
     uint32_t* __restrict__ hist, int nblocks, int items,
-    const int* __restrict__ n_dev, bool drop) {
+    const int* __restrict__ n_dev, bool drop, bool match) {
   constexpr int kBins = 1 << D;
   __shared__ uint32_t h[kBins];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -64,9 +66,13 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t key = i < n ? keys[i] : kInvalid;
     const bool valid = i < n && !(drop && key == kInvalid);
     const uint32_t d = valid ? (key >> shift) & (kBins - 1) : 0u;
-    const uint64_t peers = digit_peers<D>(d, valid);
-    if (valid && (peers & lt) == 0)
-      atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    if (match) {
+      const uint64_t peers = digit_peers<D>(d, valid);
+      if (valid && (peers & lt) == 0)
+        atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    } else if (valid) {
+      atomicAdd(&h[d], 1u);
+    }
   }
   __syncthreads();
   for (int b = tid; b < kBins; b += kThreads) hist[b * nblocks + blockIdx.x] = h[b];
@@ -192,8 +198,11 @@ void radix_pass(uint32_t* kin, uint32_t* vin, uint32_t* kout, uint32_t* vout,
                 int n, int shift, uint32_t* hist, int nblocks, int items,
                 const int* n_dev, bool drop, int* n_out, hipStream_t s) {
   uint32_t* totals = hist + (size_t)(1 << D) * nblocks;
+  // large sorts match equal digits before the LDS add (their tiles repeat
+  // a voxel's key many times); small ones are faster with one add per key
+  const bool match = n > (1 << 20);
   k_radix_hist<D><<<nblocks, kThreads, 0, s>>>(kin, n, shift, hist, nblocks,
-                                               items, n_dev, drop);
+                                               items, n_dev, drop, match);
   k_radix_rowscan<<<1 << D, 256, 0, s>>>(hist, nblocks, totals);
   k_radix_scatter<D><<<nblocks, kThreads, 0, s>>>(
       kin, vin, kout, vout, n, shift, hist, nblocks, totals, items, n_dev,
