@@ -9,25 +9,30 @@
 // serves one frame or a batch of many sensors with their own poses, ranges,
 // voxel sizes and maps.  Per sequence:
 //
-//  1 discover   one lane per point: ray set-up + voxel DDA (nm_dda.h);
-//               marks every 8^3 leaf the ray enters in the job's dense
-//               cube table around the eye, records the cell of the ray's
-//               hit voxel.
+//  1 discover   one lane per point: ray set-up (nm_dda.h), then the
+//               leaf-level form of the voxel DDA (per-axis crossing times
+//               merged a leaf at a time); marks every 8^3 leaf the ray
+//               enters in the job's dense cube table around the eye,
+//               records the cell of the ray's hit voxel.
 //  2 compact    prefix sum over all jobs' cube tables -> dense tile slots,
 //               in ascending (job, cube) order (deterministic).
-//  3 hit sort   (tile cell, point) pairs of all hits, stable radix sort:
-//               every hit voxel's hits in ascending pixel order.
+//  3 hit sort   (tile cell, point) pairs of all hits, stable radix sort
+//               (rays without a hit dropped by the first pass): every hit
+//               voxel's hits in ascending pixel order.
 //  4 segments   per hit voxel: end of its run of sorted hits into its
 //               tile (the start into hit_start), hit bit in the
 //               tile's 512-bit mask, zeroed gap counters.
-//  5 accumulate one lane per point again: integer miss counts into the
-//               tiles (LDS-aggregated per work item).  A miss in a voxel
-//               that also holds hits increments the gap counter of the
-//               first later hit (binary search in the sorted hits), so the
-//               order of hits and misses in pixel order is known exactly.
+//  5 accumulate one lane per point again, voxel DDA: integer miss counts
+//               into the tiles (LDS-aggregated per work item).  A miss in
+//               a voxel that also holds hits increments the gap counter of
+//               the first later hit (binary search in the sorted hits), so
+//               the order of hits and misses in pixel order is known
+//               exactly.
 //  6 temp       one lane per tile voxel: the temp value of each voxel is
 //               the FP32 sum of its events in pixel order, rebuilt from
-//               the counts with nm::fold (exact, any rounding mode).
+//               the counts with nm::fold (exact, any rounding mode); a
+//               wavefront per voxel with many hits (k_heavy, on a forked
+//               stream beside the per-voxel pass).
 //  7 apply      one workgroup per tile: map += temp, clamp, activity
 //               thresholds; tiles cleared.
 //
