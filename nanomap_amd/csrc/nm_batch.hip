@@ -526,7 +526,7 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
       t.queue = 0;
       cand.push_back(t);
       for (int queue = 1; queue <= 2; ++queue)
-        for (int gm : {2, 4, 8}) {
+        for (int gm : {2, 4, 6, 8}) {
           t.queue = queue;
           t.grid_mult = gm;
           cand.push_back(t);
