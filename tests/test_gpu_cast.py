@@ -327,3 +327,31 @@ def test_log_odds_and_rounding_change_between_frames(gpu):
         _run_frame(nm, ctx, gmap, omap, f, cloud)
         assert ctx.stats()["errors"] == 0
         _check_maps(gmap, omap)
+
+
+def test_stage_timing_all_and_masked(gpu):
+    """Stage timing: every stage of a frame timed once per frame, or only
+    the stages asked for; results unchanged either way."""
+    nm = gpu
+    _, cam, frames = W.cfg1(nm)
+    pos, quat, cloud = frames[0]
+    f = W.frame(nm, pos, quat, 0.1, 5.0, RD, cam.width, cam.height)
+    ctx = nm.Context(cam.width * cam.height, f.max_time)
+    maps = []
+    for stages in (None, ["accumulate"], ["discover", "update"]):
+        m = nm.Map(0.1, *W.CLAMP)
+        ctx.reset_timing()
+        ctx.set_timing(True, stages=stages)
+        for _ in range(3):
+            ctx.submit(m, f, cloud)
+        ctx.set_timing(False)
+        st = ctx.stage_times()
+        timed = {k for k, (ms, n) in st.items() if n}
+        want = set(nm.Context.STAGES) - {"filter"} if stages is None else set(stages)
+        assert timed == want, (stages, st)
+        assert all(n == 3 and ms > 0 for k, (ms, n) in st.items() if k in want)
+        maps.append(m.download())
+    for o, v, mk in maps[1:]:
+        assert np.array_equal(o, maps[0][0])
+        assert np.array_equal(v.view(np.uint32), maps[0][1].view(np.uint32))
+        assert np.array_equal(mk, maps[0][2])
