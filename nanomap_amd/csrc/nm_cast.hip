@@ -1087,14 +1087,29 @@ constexpr uint32_t kHitsMask = kTouchedBit - 1;
 // are listed for k_heavy, which gives each its own wavefront.  The result
 // replaces the voxel's miss count in the tile.
 // miss_runs[job][k] = fold(0, miss, k): the sum of a voxel with k misses
-// and no hit.
+// and no hit.  A job whose table already holds its (miss, rounding) --
+// runs_key, set by k_runs_mark after the table is built -- is skipped, so a
+// batch replayed with unchanged log-odds rebuilds nothing.
+__device__ __forceinline__ uint64_t runs_key_of(float miss, int M) {
+  return ((uint64_t)bits_of(miss) << 8) | (uint64_t)M;
+}
+
 template <int M>
-__global__ __launch_bounds__(256) void k_miss_runs(const Job* __restrict__ jobs,
-                                                   float* __restrict__ runs) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_miss_runs(
+    const Job* __restrict__ jobs, float* __restrict__ runs,
+    const uint64_t* __restrict__ runs_key) {
   const float miss = jobs[blockIdx.y].miss;
+  if (runs_key[blockIdx.y] == runs_key_of(miss, M)) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
   if (k < kMissRuns)
     runs[(size_t)blockIdx.y * kMissRuns + k] = fold<M, true>(0.0f, miss, k);
+}
+
+template <int M>
+__global__ void k_runs_mark(const Job* __restrict__ jobs, int njobs,
+                            uint64_t* __restrict__ runs_key) {
+  for (int j = threadIdx.x; j < njobs; j += blockDim.x)
+    runs_key[j] = runs_key_of(jobs[j].miss, M);
 }
 
 template <int M>
@@ -1533,10 +1548,15 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
                                    b.map_new);
   const dim3 rgrid(kMissRuns / 256, p.njobs);
   if (!p.miss_runs_ready) {
-    if (down)
-      k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
-    else
-      k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs);
+    if (down) {
+      k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
+                                               b.runs_key);
+      k_runs_mark<kDown><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
+    } else {
+      k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
+                                                  b.runs_key);
+      k_runs_mark<kNearest><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
+    }
   }
   if (down)
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,

@@ -151,6 +151,7 @@ struct Workspace {
   float* filter_tmp;    // [ray_cap * 4]
   int32_t* flags;       // [ray_cap]
   float* miss_runs;     // [max_jobs * kMissRuns]  fold(0, miss, k) per job
+  uint64_t* runs_key;   // [max_jobs]  (miss bits, rounding) each table holds
 };
 
 // Sizes of a workspace.

@@ -51,6 +51,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.item_job, I));
   A(dalloc(&b.item_order, I));
   A(dalloc(&b.miss_runs, J * kMissRuns));
+  A(dalloc(&b.runs_key, J));
   if (d.host_clouds) {
     A(dalloc(&b.cloud, P * 16));
     A(dalloc(&b.filter_rays, P * 4));
@@ -62,6 +63,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   if (e == hipSuccess)
     e = hipMemset(b.hit_mask, 0, T * kMaskWords * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemset(b.counters, 0, sizeof(FrameCounters));
+  if (e == hipSuccess) e = hipMemset(b.runs_key, 0xff, J * sizeof(uint64_t));
   if (e == hipSuccess)
     e = hipMemset(b.scan_job, 0, scan_blocks * sizeof(int32_t));
   if (e == hipSuccess) e = hipMemset(b.item_job, 0, I * sizeof(int32_t));
@@ -83,7 +85,8 @@ void workspace_free(Workspace* b) {
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
                   b->jobs,       b->maps,        b->part_discover,
                   b->item_job,   b->item_order,  b->cloud,      b->filter_rays,
-                  b->filter_tmp, b->flags,       b->miss_runs};
+                  b->filter_tmp, b->flags,       b->miss_runs,
+                  b->runs_key};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   *b = Workspace{};
