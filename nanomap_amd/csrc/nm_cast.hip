@@ -293,16 +293,20 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
       for (;;) {
         const double t = fmin_(ax.t, fmin_(ay.t, az.t));
         if (!(t <= L)) break;
-        if (az.t == t) {
-          lz += 8 * s.sz;
-          axis_leaf<M>(az, L);
-        } else if (ay.t == t) {
-          ly += 8 * s.sy;
-          axis_leaf<M>(ay, L);
-        } else {
-          lx += 8 * s.sx;
-          axis_leaf<M>(ax, L);
-        }
+        // the axis is chosen by selects, and one axis_leaf serves all lanes
+        const bool bz = az.t == t, by = !bz && ay.t == t, bx = !bz && !by;
+        AxisRun a;
+        a.t = t;
+        a.d = bz ? az.d : (by ? ay.d : ax.d);
+        a.over = bz ? az.over : (by ? ay.over : ax.over);
+        a.n = bz ? az.n : (by ? ay.n : ax.n);
+        axis_leaf<M>(a, L);
+        if (bx) ax = a;
+        if (by) ay = a;
+        if (bz) az = a;
+        lx += bx ? 8 * s.sx : 0;
+        ly += by ? 8 * s.sy : 0;
+        lz += bz ? 8 * s.sz : 0;
         c = cube_index(J.cube, lx, ly, lz);
         if (c < 0)
           ++errors;
