@@ -54,6 +54,14 @@ extern "C" {
 
 #define NM_FILTER_NONE 0  /* every point is a ray: Handler::populateTempGrid */
 #define NM_FILTER_VOXEL 1 /* voxel-centroid filter: filterCloud.cu:84,279   */
+/* The voxel filter's lower-memory variants (PrecisionType 1-3,
+ * filterCloud.cu:134-277, 319-434), with the sums taken in ascending pixel
+ * order instead of by atomics: */
+#define NM_FILTER_VOXEL_HALF 2   /* centroid from FP16 sums and FP16 count  */
+#define NM_FILTER_VOXEL_COUNT 3  /* voxel centre, int count                 */
+#define NM_FILTER_VOXEL_COUNT8 4 /* voxel centre, 8-bit count: a voxel whose
+                                    count is a multiple of 256 wraps to 0
+                                    and emits no ray                        */
 
 typedef struct nm_ctx nm_ctx; /* per-sensor device workspace             */
 typedef struct nm_map nm_map; /* device-resident sparse log-odds map     */
@@ -156,8 +164,8 @@ int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out);
 int nm_counts_download(nm_ctx* ctx, int64_t max_leaves, int64_t* n_leaves,
                        int32_t* origins, uint32_t* miss, uint32_t* hit);
 
-/* Rays produced by the voxel filter of the last nm_cast with
- * filter = NM_FILTER_VOXEL: (dir xyz, length) per ray, index units.
+/* Rays produced by the voxel filter of the last nm_cast with a voxel
+ * filter (NM_FILTER_VOXEL*): (dir xyz, length) per ray, index units.
  * Synchronous. */
 int nm_filter_rays_download(nm_ctx* ctx, int max_rays, int* n_rays,
                             float* rays4);

@@ -10,7 +10,9 @@ import ctypes
 
 import numpy as np
 
-from ._capi import (FILTER_NONE, FILTER_VOXEL, ROUND_DOWN, ROUND_NEAREST,
+from ._capi import (FILTER_NONE, FILTER_VOXEL, FILTER_VOXEL_COUNT,
+                    FILTER_VOXEL_COUNT8, FILTER_VOXEL_HALF, ROUND_DOWN,
+                    ROUND_NEAREST,
                     AgentFrame, BatchConfig, BatchTuning, Camera, CtxConfig,
                     Frame, FrameStats, NanoMapError, SimSensor, _ptr, check,
                     lib)
@@ -18,7 +20,8 @@ from ._capi import (FILTER_NONE, FILTER_VOXEL, ROUND_DOWN, ROUND_NEAREST,
 __all__ = ["Context", "Map", "Batch", "Scene", "Camera", "Frame", "FrameStats",
            "SimGrid", "SimSensor", "sim_sensor",
            "frame_from_pose", "fold", "NanoMapError", "ROUND_DOWN",
-           "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "DEFAULT_F"]
+           "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "FILTER_VOXEL_HALF",
+           "FILTER_VOXEL_COUNT", "FILTER_VOXEL_COUNT8", "DEFAULT_F"]
 
 # FrameTransform of the reference's sample sensor (config/sensors/
 # 640x480.txt:22-25, column i = file line i): optical z -> body x.

@@ -21,7 +21,10 @@ NM_ESTATE = 6
 ROUND_NEAREST = 0
 ROUND_DOWN = 1
 FILTER_NONE = 0
-FILTER_VOXEL = 1
+FILTER_VOXEL = 1         # FP32 centroid (PrecisionType 0)
+FILTER_VOXEL_HALF = 2    # FP16 sums and count (PrecisionType 1)
+FILTER_VOXEL_COUNT = 3   # voxel centre, int count (PrecisionType 2)
+FILTER_VOXEL_COUNT8 = 4  # voxel centre, 8-bit count (PrecisionType 3)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libnanomap_amd.so")

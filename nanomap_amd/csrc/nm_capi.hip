@@ -389,7 +389,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     return fail(NM_EINVAL, "nm_cast: log-odds must be finite, magnitude <= 1e27");
   if (frame->rounding != NM_ROUND_NEAREST && frame->rounding != NM_ROUND_DOWN)
     return fail(NM_EINVAL, "nm_cast: bad rounding mode");
-  if (frame->filter != NM_FILTER_NONE && frame->filter != NM_FILTER_VOXEL)
+  if (frame->filter < NM_FILTER_NONE || frame->filter > NM_FILTER_VOXEL_COUNT8)
     return fail(NM_EINVAL, "nm_cast: bad filter type");
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(frame->eye[a]) || fabs(frame->eye[a]) > 1e8)
@@ -435,7 +435,7 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
                 ? 1
                 : 0;
   p.layout = nm::kLayout16x16;
-  p.use_rays = frame->filter == NM_FILTER_VOXEL ? 1 : 0;
+  p.use_rays = frame->filter != NM_FILTER_NONE ? 1 : 0;
   p.rays4 = p.use_rays ? ctx->b.filter_rays : nullptr;
   p.n_dev = p.use_rays ? &ctx->b.counters->n_filter_rays : nullptr;
   p.map = 0;
@@ -462,12 +462,13 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   ctx->job = p;
 
   NM_HIP(nm::launch_begin(ctx->b, &p, s));
-  if (frame->filter == NM_FILTER_VOXEL) {
+  if (frame->filter != NM_FILTER_NONE) {
     StageTimer t(ctx, NM_STAGE_FILTER, s);
     float origin[3];
     for (int a = 0; a < 3; ++a)
       origin[a] = (float)(frame->eye[a] - floor(frame->eye[a]));
-    NM_HIP(nm::launch_voxel_filter(p, origin, ctx->b, s));
+    NM_HIP(nm::launch_voxel_filter(p, origin, frame->filter - NM_FILTER_VOXEL,
+                                   ctx->b, s));
   }
   {
     StageTimer t(ctx, NM_STAGE_DISCOVER, s);

@@ -1,7 +1,11 @@
 // nm_filter.hip -- deterministic voxel filter.
 //
 // Replaces voxelFilter + rayFromFilter (FP32 mode, PrecisionType 0,
-// src/nanomap/kernels/filterCloud.cu:84-133, 279-318).  The reference bins
+// src/nanomap/kernels/filterCloud.cu:84-133, 279-318) and its lower-memory
+// variants (PrecisionType 1-3, filterCloud.cu:134-277, 319-434): FP16
+// sums and count (voxelFilterHalf2/rayFromFilterHalf2), and voxel-centre
+// rays from an int count (Simple) or an 8-bit count that wraps
+// (SimpleQuad: 256 points make a count of 0 and no ray).  The reference bins
 // points with float atomicAdd into a dense buffer over the frustum's voxel
 // bounding box (about 273 MB cleared and scanned per 640x480 frame at
 // 0.05 m) and appends rays through an atomic counter, so both the sums
@@ -11,8 +15,9 @@
 //           32-bit key in (x, y, z) order; value = pixel index;
 //   sort    stable radix sort of (key, pixel) -> each bin's points are
 //           contiguous and in ascending pixel order;
-//   bins    one lane per bin: FP32 sum of frac(r/res) in pixel order,
-//           centroid, ray (dir, length) if 0 < length < max_time;
+//   bins    one lane per bin: FP32 (or FP16) sum of frac(r/res) in pixel
+//           order, centroid (or the voxel centre), ray (dir, length) if
+//           0 < length < max_time;
 //   emit    prefix sum of the kept flags -> rays in ascending key order.
 // Memory is O(points), independent of range and voxel size.
 #include "nm_internal.h"
@@ -78,8 +83,10 @@ __global__ __launch_bounds__(256) void k_filter_keys(
     atomicAdd(&cnt->n_filter_points, (int)__popcll(m));
 }
 
+// precision: 0 FP32 centroid, 1 FP16 centroid, 2 voxel centre (int
+// count), 3 voxel centre (8-bit count).
 __global__ __launch_bounds__(256) void k_filter_bins(
-    CastParams p, float o0, float o1, float o2, FilterKey fk,
+    CastParams p, float o0, float o1, float o2, FilterKey fk, int precision,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const FrameCounters* __restrict__ cnt, float4* __restrict__ tmp,
     int32_t* __restrict__ flags) {
@@ -90,28 +97,53 @@ __global__ __launch_bounds__(256) void k_filter_bins(
     int keep = 0;
     if (e < nv && (e == 0 || keys[e - 1] != keys[e])) {
       const uint32_t key = keys[e];
-      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-      uint32_t n = 0;
-      for (int q = e; q < nv && keys[q] == key; ++q) {
-        bool valid;
-        int b[3];
-        float f[3];
-        point_bin(p, origin, (int)vals[q], valid, b, f);
-        sx += f[0];
-        sy += f[1];
-        sz += f[2];
-        ++n;
-      }
       const uint32_t m = (1u << fk.bits) - 1;
       const int bx = (int)((key >> (2 * fk.bits)) & m) - fk.offset;
       const int by = (int)((key >> fk.bits) & m) - fk.offset;
       const int bz = (int)(key & m) - fk.offset;
-      const float c = (float)n;
-      const float pt[3] = {sx / c + (float)bx, sy / c + (float)by,
-                           sz / c + (float)bz};
+      float pt[3];
+      bool emit = true;
+      if (precision <= 1) {
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        _Float16 hx = 0, hy = 0, hz = 0, hn = 0;
+        uint32_t n = 0;
+        for (int q = e; q < nv && keys[q] == key; ++q) {
+          bool valid;
+          int b[3];
+          float f[3];
+          point_bin(p, origin, (int)vals[q], valid, b, f);
+          if (precision == 0) {
+            sx += f[0];
+            sy += f[1];
+            sz += f[2];
+          } else {  // each term rounded to FP16, each sum in FP16
+            hx = hx + (_Float16)f[0];
+            hy = hy + (_Float16)f[1];
+            hz = hz + (_Float16)f[2];
+            hn = hn + (_Float16)1.0f;
+          }
+          ++n;
+        }
+        if (precision == 1) {
+          sx = (float)hx;
+          sy = (float)hy;
+          sz = (float)hz;
+        }
+        const float c = precision == 0 ? (float)n : (float)hn;
+        pt[0] = sx / c + (float)bx;
+        pt[1] = sy / c + (float)by;
+        pt[2] = sz / c + (float)bz;
+      } else {
+        uint32_t n = 0;
+        for (int q = e; q < nv && keys[q] == key; ++q) ++n;
+        emit = precision == 2 || (n & 255u) != 0u;
+        pt[0] = (float)bx + 0.5f;
+        pt[1] = (float)by + 0.5f;
+        pt[2] = (float)bz + 0.5f;
+      }
       const float len =
           __builtin_sqrtf(pt[0] * pt[0] + pt[1] * pt[1] + pt[2] * pt[2]);
-      if (len > 0.0f && len < p.max_time) {
+      if (emit && len > 0.0f && len < p.max_time) {
         tmp[e] = make_float4(pt[0] / len, pt[1] / len, pt[2] / len, len);
         keep = 1;
       }
@@ -199,7 +231,8 @@ __global__ __launch_bounds__(256) void k_flag_emit(
 }  // namespace
 
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               const Workspace& b, hipStream_t s) {
+                               int precision, const Workspace& b,
+                               hipStream_t s) {
   if (p.n <= 0) return hipSuccess;
   FilterKey fk;
   const int need = (int)__builtin_ceilf(p.max_time) + 3;
@@ -217,7 +250,8 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
                                   &out, s);
   if (e != hipSuccess) return e;
   k_filter_bins<<<min(grid, 4096), 256, 0, s>>>(
-      p, origin[0], origin[1], origin[2], fk, b.keys[out], b.vals[out],
+      p, origin[0], origin[1], origin[2], fk, precision, b.keys[out],
+      b.vals[out],
       b.counters, reinterpret_cast<float4*>(b.filter_tmp), b.flags);
   const int nblocks = (p.n + kScanChunk - 1) / kScanChunk;
   k_flag_count<<<nblocks, 256, 0, s>>>(b.flags, p.n, b.scan_blocks);

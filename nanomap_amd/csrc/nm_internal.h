@@ -250,7 +250,8 @@ int radix_hist_words(int n);  // histogram buffer size for n keys
 // part of the eye, eye - floor(eye), as FP32 (the reference's
 // voxelOriginOffset, include/nanomap/sensor/SensorData.h:55).
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               const Workspace& b, hipStream_t s);
+                               int precision, const Workspace& b,
+                               hipStream_t s);
 
 void set_error(const std::string& msg);
 

@@ -59,7 +59,11 @@ class Manager {
                                NM_ROUND_DOWN, &f));
       f.width = pcl_width;
       f.height = pcl_height;
-      f.filter = _config->filterType() == 1 ? NM_FILTER_VOXEL : NM_FILTER_NONE;
+      // FilterType 1 with PrecisionType 0-3 (filterCloud.cu:476-553).
+      const int prec = _config->precisionType();
+      f.filter = _config->filterType() == 1 && prec >= 0 && prec <= 3
+                     ? NM_FILTER_VOXEL + prec
+                     : NM_FILTER_NONE;
       const int n = pcl_width * pcl_height;
       nm_ctx* ctx = context(*s, n, f.max_time);
       check(nm_frame_submit(ctx, map->handle(), &f, cloud, n, pcl_step, 0,

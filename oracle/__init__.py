@@ -40,7 +40,7 @@ SIGNATURES = {
     "or_dda_trace": (_I, [_PD, _PF, _F, _F, _I, _PI32, _I,
                           ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "or_fold": (_F, [_F, _F, ctypes.c_uint32, _I]),
-    "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _PF, _I]),
+    "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _PF, _I, _I]),
     "or_simgrid_build": (_P, [_PI32, _PF, ctypes.c_int64, _F]),
     "or_simgrid_destroy": (None, [_P]),
     "or_simgrid_active_count": (ctypes.c_int64, [_P]),
@@ -191,7 +191,11 @@ def fold(v, a, k, rounding):
     return lib().or_fold(float(v), float(a), int(k), int(rounding))
 
 
-def voxel_filter(frame, cloud, max_rays=None):
+def voxel_filter(frame, cloud, max_rays=None, precision=None):
+    """Rays of the voxel filter; precision 0-3 as PrecisionType (default:
+    from frame.filter, NM_FILTER_VOXEL.. -> 0..3)."""
+    if precision is None:
+        precision = max(int(getattr(frame, "filter", 1)) - 1, 0)
     a = np.ascontiguousarray(cloud, dtype=np.float32)
     a = a.reshape(a.shape[0], -1) if a.size else a.reshape(0, 3)
     R, eye = _frame_args(frame)
@@ -200,7 +204,7 @@ def voxel_filter(frame, cloud, max_rays=None):
     n = lib().or_voxel_filter(a.ctypes.data_as(_P), a.shape[0], a.shape[1] * 4,
                               _p(R, ctypes.c_float), _p(eye, ctypes.c_double),
                               frame.grid_res, frame.max_time,
-                              _p(out, ctypes.c_float), max_rays)
+                              _p(out, ctypes.c_float), max_rays, precision)
     return out[:min(n, max_rays)]
 
 

@@ -432,13 +432,35 @@ float or_fold(float v0, float a, uint32_t k, int rounding) {
 // Validity: no NaN coordinate and 0 < |p| < max range (max_time * res).
 // Bins are relative to floor(eye) and emitted in ascending
 // (x, y, z) order.
+//
+// precision (PrecisionType, filterCloud.cu:476-553): 0 the FP32 centroid
+// above; 1 the same with each fraction rounded to FP16 and every sum,
+// count included, rounded to FP16 (voxelFilterHalf2 / rayFromFilterHalf2,
+// filterCloud.cu:134-182, 319-351); 2 the voxel centre bin + 0.5 for every
+// occupied bin (voxelFilterSimple / rayFromFilterSimple, 183-224,
+// 352-384); 3 the same with an 8-bit count, so a bin holding a multiple of
+// 256 points wraps to 0 and emits no ray (SimpleQuad, 225-277, 385-434).
+
+// x rounded to the nearest FP16 value, ties to even (subnormals included;
+// the values here stay far below the FP16 maximum).
+static float half_rn(double x) {
+  if (x == 0.0) return (float)x;
+  int e = 0;
+  std::frexp(std::fabs(x), &e);  // |x| in [2^(e-1), 2^e)
+  const int q = (e - 1 < -14 ? -14 : e - 1) - 10;  // exponent of the ulp
+  const double u = std::ldexp(1.0, q);
+  return (float)(std::nearbyint(x / u) * u);  // exact scaling; RN mode
+}
+
 int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     const float R[9], const double eye[3], float res,
-                    float max_time, float* rays4, int max_rays) {
+                    float max_time, float* rays4, int max_rays,
+                    int precision) {
   RoundGuard guard(0);
   struct Bin {
     float sx, sy, sz;
     uint32_t n;
+    float hx, hy, hz, hn;  // FP16 sums (precision 1)
   };
   const float max_range = max_time * res;
   float origin[3];
@@ -479,14 +501,19 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
     auto it = index.find(key);
     if (it == index.end()) {
       index.emplace(key, bins.size());
-      bins.push_back({Key{b[0], b[1], b[2]}, Bin{f[0], f[1], f[2], 1u}});
-    } else {
-      Bin& bin = bins[it->second].second;
-      bin.sx += f[0];
-      bin.sy += f[1];
-      bin.sz += f[2];
-      bin.n += 1;
+      bins.push_back({Key{b[0], b[1], b[2]},
+                      Bin{0.0f, 0.0f, 0.0f, 0u, 0.0f, 0.0f, 0.0f, 0.0f}});
+      it = index.find(key);
     }
+    Bin& bin = bins[it->second].second;
+    bin.sx += f[0];
+    bin.sy += f[1];
+    bin.sz += f[2];
+    bin.n += 1;
+    bin.hx = half_rn((double)bin.hx + (double)half_rn(f[0]));
+    bin.hy = half_rn((double)bin.hy + (double)half_rn(f[1]));
+    bin.hz = half_rn((double)bin.hz + (double)half_rn(f[2]));
+    bin.hn = half_rn((double)bin.hn + 1.0);
   }
   std::sort(bins.begin(), bins.end(),
             [](const std::pair<Key, Bin>& a, const std::pair<Key, Bin>& b) {
@@ -494,10 +521,19 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
             });
   int n = 0;
   for (const auto& kb : bins) {
-    const float cnt = (float)kb.second.n;
-    float pt[3] = {kb.second.sx / cnt + (float)kb.first.x,
-                   kb.second.sy / cnt + (float)kb.first.y,
-                   kb.second.sz / cnt + (float)kb.first.z};
+    const Bin& bn = kb.second;
+    float pt[3];
+    if (precision <= 1) {
+      const float cnt = precision == 0 ? (float)bn.n : bn.hn;
+      pt[0] = (precision == 0 ? bn.sx : bn.hx) / cnt + (float)kb.first.x;
+      pt[1] = (precision == 0 ? bn.sy : bn.hy) / cnt + (float)kb.first.y;
+      pt[2] = (precision == 0 ? bn.sz : bn.hz) / cnt + (float)kb.first.z;
+    } else {
+      if (precision == 3 && bn.n % 256u == 0u) continue;
+      pt[0] = (float)kb.first.x + 0.5f;
+      pt[1] = (float)kb.first.y + 0.5f;
+      pt[2] = (float)kb.first.z + 0.5f;
+    }
     const float len = std::sqrt(pt[0] * pt[0] + pt[1] * pt[1] + pt[2] * pt[2]);
     if (!(len > 0.0f && len < max_time)) continue;
     if (n < max_rays) {

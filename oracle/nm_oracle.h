@@ -80,7 +80,8 @@ float or_fold(float v0, float a, uint32_t k, int rounding);
  * order.  Returns the number of rays written (<= max_rays). */
 int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     const float R[9], const double eye[3], float res,
-                    float max_time, float* rays4, int max_rays);
+                    float max_time, float* rays4, int max_rays,
+                    int precision);
 
 /* ---- sensor-simulation cast (generateCloud.cu:20-82, Sensor.h:156-227) - */
 typedef struct or_simgrid or_simgrid;

@@ -45,7 +45,7 @@ def _run_frame(nm, ctx, gmap, omap, frame, cloud, check_counts=True):
              frame.filter), flush=True)
     ctx.cast(frame, cloud)
     temp = oracle.Temp()
-    if frame.filter == nm.FILTER_VOXEL:
+    if frame.filter != nm.FILTER_NONE:
         rays = oracle.voxel_filter(frame, cloud)
         temp.cast_rays(frame, rays)
     else:
@@ -218,6 +218,53 @@ def test_voxel_filter_bit_exact(gpu, rounding):
         ctx.update(gmap)
         omap.integrate(temp, rounding)
     _check_maps(gmap, omap)
+
+
+@pytest.mark.parametrize("mode", ["half", "count", "count8"])
+def test_voxel_filter_precision_modes_bit_exact(gpu, mode):
+    # PrecisionType 1-3: two sensor frames through the whole update, then a
+    # clustered cloud with 256-, 300- and 2100-point voxels (the 8-bit count
+    # wraps to 0 at 256; the FP16 count stops at 2048).
+    nm = gpu
+    filt = {"half": nm.FILTER_VOXEL_HALF, "count": nm.FILTER_VOXEL_COUNT,
+            "count8": nm.FILTER_VOXEL_COUNT8}[mode]
+    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, filt=filt)
+        ctx.cast(f, cloud)
+        g_rays = ctx.filter_rays()
+        o_rays = oracle.voxel_filter(f, cloud)
+        assert g_rays.shape == o_rays.shape and len(o_rays) > 1000
+        assert np.array_equal(g_rays.view(np.uint32), o_rays.view(np.uint32))
+        temp = oracle.Temp()
+        temp.cast_rays(f, o_rays)
+        _check_counts(ctx, temp)
+        ctx.update(gmap)
+        omap.integrate(temp, RD)
+    _check_maps(gmap, omap)
+
+    rng = np.random.default_rng(23)
+    f = W.frame(nm, (1.23, -0.47, 0.22), (1.0, 0.0, 0.0, 0.0), 0.1, 2.0,
+                RD, filt=filt)
+    origin = np.array([e - np.floor(e) for e in f.eye])
+    R = np.asarray(f.rotation, np.float64).reshape(3, 3)  # sensor -> world
+    parts = [rng.normal(size=(400, 3)).astype(np.float32) * np.float32(0.6)]
+    for n, k in ((256, (14, 8, 5)), (300, (-3, 4, 3)), (2100, (2, -4, -1))):
+        c = (np.asarray(k) + 0.5 - origin) * 0.1  # that voxel's centre
+        w = c + rng.uniform(-0.03, 0.03, size=(n, 3))
+        parts.append((w @ R).astype(np.float32))  # R^T w, row by row
+    cloud = np.concatenate(parts)
+    print("clustered cloud: points=%d filter=%d" % (len(cloud), filt), flush=True)
+    ctx.cast(f, cloud)
+    g_rays = ctx.filter_rays()
+    o_rays = oracle.voxel_filter(f, cloud)
+    assert g_rays.shape == o_rays.shape and len(o_rays) > 50
+    assert np.array_equal(g_rays.view(np.uint32), o_rays.view(np.uint32))
+    if mode == "count8":
+        assert len(o_rays) == len(oracle.voxel_filter(f, cloud, precision=2)) - 1
 
 
 def test_overflow_is_reported_not_fatal(gpu):

@@ -290,7 +290,11 @@ def test_duplicate_rays_sum_in_pixel_order(oracle_lib):
 
 # ------------------------------------------------------------ filter ----
 
-def _filter_numpy(cloud, R, eye, res, max_time):
+def _filter_numpy(cloud, R, eye, res, max_time, precision=0):
+    """Ascending-key rays; precision 1 sums FP16-rounded fractions and the
+    count in numpy's float16 (correctly rounded: float32 holds the exact
+    sum of two halves' 11-bit significands to 24 bits, so rounding twice
+    is innocuous), 2 and 3 give voxel centres, 3 with an 8-bit count."""
     R = np.asarray(R, np.float32).reshape(3, 3)
     origin = np.array([np.float32(e - math.floor(e)) for e in eye], np.float32)
     bins = {}
@@ -310,16 +314,27 @@ def _filter_numpy(cloud, R, eye, res, max_time):
             fr.append(np.float32(q - np.float32(math.floor(q))))
         k = tuple(key)
         if k not in bins:
-            bins[k] = [np.float32(0)] * 3 + [0]
+            bins[k] = [np.float32(0)] * 3 + [0] + [np.float16(0)] * 4
         b = bins[k]
         for a in range(3):
             b[a] = np.float32(b[a] + fr[a])
+            b[4 + a] = np.float16(b[4 + a] + np.float16(fr[a]))
         b[3] += 1
+        b[7] = np.float16(b[7] + np.float16(1))
     out = []
     for k in sorted(bins):
         b = bins[k]
-        c = np.float32(b[3])
-        pt = [np.float32(np.float32(b[a] / c) + np.float32(k[a])) for a in range(3)]
+        if precision == 0:
+            c = np.float32(b[3])
+            pt = [np.float32(np.float32(b[a] / c) + np.float32(k[a])) for a in range(3)]
+        elif precision == 1:
+            c = np.float32(b[7])
+            pt = [np.float32(np.float32(np.float32(b[4 + a]) / c) + np.float32(k[a]))
+                  for a in range(3)]
+        else:
+            if precision == 3 and b[3] % 256 == 0:
+                continue
+            pt = [np.float32(np.float32(k[a]) + np.float32(0.5)) for a in range(3)]
         ln = np.sqrt(np.float32(np.float32(pt[0] * pt[0] + pt[1] * pt[1]) + pt[2] * pt[2]))
         if ln > 0 and ln < np.float32(max_time):
             out.append([pt[0] / ln, pt[1] / ln, pt[2] / ln, ln])
@@ -338,6 +353,43 @@ def test_voxel_filter_matches_numpy_restatement(oracle_lib):
     exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time)
     assert got.shape == exp.shape and got.shape[0] > 50
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
+def test_voxel_filter_precision_modes(oracle_lib, precision):
+    # Scattered points plus dense voxels: 256 points (the 8-bit count
+    # wraps to 0), 300, and 2100 (the FP16 count stops at 2048).
+    rng = np.random.default_rng(17 + precision)
+    f = _F([12.3, -4.7, 2.2], res=0.1, max_time=20.0)
+    origin = np.array([e - math.floor(e) for e in f.eye])
+    parts = [rng.normal(size=(400, 3)).astype(np.float32) * np.float32(0.6)]
+    for n, k in ((256, (14, 8, 5)), (300, (-3, 4, 3)), (2100, (2, -4, -1))):
+        c = (np.asarray(k) + 0.5 - origin) * 0.1  # that voxel's centre
+        parts.append((c + rng.uniform(-0.03, 0.03, size=(n, 3))).astype(np.float32))
+    cloud = np.concatenate(parts)
+    got = oracle.voxel_filter(f, cloud, precision=precision)
+    exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time,
+                        precision)
+    assert got.shape == exp.shape and got.shape[0] > 50
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    if precision == 1:  # FP16 sums differ from FP32 ones in some bins
+        ref = oracle.voxel_filter(f, cloud, precision=0)
+        assert ref.shape == got.shape and not np.array_equal(ref, got)
+    if precision == 3:  # exactly the 256-point voxel is dropped
+        assert got.shape[0] == oracle.voxel_filter(f, cloud,
+                                                   precision=2).shape[0] - 1
+
+
+def test_half_rounding_matches_numpy_float16(oracle_lib):
+    # Subnormal, tie and saturating-count cases of the FP16 sums, checked
+    # through single-point and repeated-point voxels.
+    f = _F([0.5, 0.5, 0.5], res=1.0, max_time=20.0)
+    for x in (1.0 + 2.0 ** -12, 1.0 + 3 * 2.0 ** -12, 1.0 + 2.0 ** -20,
+              1.5 + 2.0 ** -11, 2.0 + 1e-7):
+        cloud = np.array([[x, 0.0, 0.0]] * 3, np.float32)
+        got = oracle.voxel_filter(f, cloud, precision=1)
+        exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time, 1)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), x
 
 
 def test_frame_cast_totals_match_ray_traces(oracle_lib):
