@@ -198,6 +198,27 @@ __device__ __forceinline__ void block_partials(uint64_t* __restrict__ out,
 // number of crossings at or below L, the end voxel is the start voxel moved
 // by that many crossings per axis, and the exit time is the first crossing
 // above L.
+// FP64 rounding mode of the wave (MODE.FP_ROUND bits 3:2, shared by FP64
+// and FP16) around the round-down voxel loop: toward -inf inside it, so
+// each crossing time c + delta is one v_add_f64, and round-to-nearest
+// again after it, for the emulated set-up of the next ray.  The loop's only
+// FP operation is that add (the rest is integer and LDS work).  The DDA
+// state passes through the asm statement that sets the mode, so the
+// round-to-nearest code producing it completes before the change and the
+// loop's adds cannot start ahead of it; the loop exits only through the
+// statement that restores the mode.  The s_nop covers the mode write's
+// latency before the first dependent VALU instruction.
+__device__ __forceinline__ void fp64_round_down_begin(Dda& s, double& L) {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 2\n\ts_nop 3"
+               : "+v"(s.nx), "+v"(s.ny), "+v"(s.nz), "+v"(s.dx), "+v"(s.dy),
+                 "+v"(s.dz), "+v"(L));
+}
+
+__device__ __forceinline__ void fp64_round_nearest() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0\n\ts_nop 3" ::
+                   : "memory");
+}
+
 struct AxisRun {
   double t;     // time of the pending leaf-entering crossing
   double d;     // time between crossings
@@ -205,10 +226,12 @@ struct AxisRun {
   int n;        // crossings at or below L produced so far
 };
 
-template <int M>
+// kHwDown: the wave's FP64 rounding mode is toward -inf (the round-down
+// sum is the hardware add).
+template <int M, bool kHwDown>
 __device__ __forceinline__ void axis_steps(AxisRun& a, int k, double L) {
   for (int i = 0; i < k; ++i) {
-    a.t = M == kDown ? add_down_pos(a.t, a.d) : a.t + a.d;
+    a.t = (M == kDown && !kHwDown) ? add_down_pos(a.t, a.d) : a.t + a.d;
     if (a.t <= L)
       ++a.n;
     else if (a.over == 1.7976931348623157e308)
@@ -216,8 +239,9 @@ __device__ __forceinline__ void axis_steps(AxisRun& a, int k, double L) {
   }
 }
 
-// The next leaf-entering crossing: eight crossings on.
-template <int M>
+// The next leaf-entering crossing: eight crossings on.  The in-binade jump
+// uses only exact operations, so it is the same in either rounding mode.
+template <int M, bool kHwDown>
 __device__ __forceinline__ void axis_leaf(AxisRun& a, double L) {
   const uint32_t e = (uint32_t)(bits_of(a.t) >> 52);  // t >= 0: exponent
   if (e > 52u && e < 0x7feu) {
@@ -237,12 +261,12 @@ __device__ __forceinline__ void axis_leaf(AxisRun& a, double L) {
       }
     }
   }
-  axis_steps<M>(a, 8, L);
+  axis_steps<M, kHwDown>(a, 8, L);
 }
 
 // First leaf-entering crossing of an axis from its first crossing time c1:
 // crossing k moves the voxel from v0 + s(k - 1) to v0 + s k.
-template <int M>
+template <int M, bool kHwDown>
 __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
                                               int v0, double L) {
   AxisRun a;
@@ -256,7 +280,7 @@ __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
   else
     a.over = c1;
   const int k0 = s > 0 ? 8 - (v0 & 7) : (v0 & 7) + 1;
-  axis_steps<M>(a, k0 - 1, L);
+  axis_steps<M, kHwDown>(a, k0 - 1, L);
   return a;
 }
 
@@ -279,10 +303,12 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
       valid = 1;
       Dda s;
       dda_init<M>(J.eye, r, s);
-      const double L = r.L;
-      AxisRun ax = axis_start<M>(s.nx, s.dx, s.sx, s.vx, L);
-      AxisRun ay = axis_start<M>(s.ny, s.dy, s.sy, s.vy, L);
-      AxisRun az = axis_start<M>(s.nz, s.dz, s.sz, s.vz, L);
+      double L = r.L;
+      constexpr bool kHw = M == kDown;
+      if (kHw) fp64_round_down_begin(s, L);
+      AxisRun ax = axis_start<M, kHw>(s.nx, s.dx, s.sx, s.vx, L);
+      AxisRun ay = axis_start<M, kHw>(s.ny, s.dy, s.sy, s.vy, L);
+      AxisRun az = axis_start<M, kHw>(s.nz, s.dz, s.sz, s.vz, L);
       // a voxel of the current leaf per axis (only v >> 3 is used)
       int lx = s.vx, ly = s.vy, lz = s.vz;
       int c = cube_index(J.cube, lx, ly, lz);
@@ -300,7 +326,7 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
         a.d = bz ? az.d : (by ? ay.d : ax.d);
         a.over = bz ? az.over : (by ? ay.over : ax.over);
         a.n = bz ? az.n : (by ? ay.n : ax.n);
-        axis_leaf<M>(a, L);
+        axis_leaf<M, kHw>(a, L);
         if (bx) ax = a;
         if (by) ay = a;
         if (bz) az = a;
@@ -313,6 +339,7 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
         else
           ls[c] = -2;
       }
+      if (kHw) fp64_round_nearest();
       visits = (uint32_t)(ax.n + ay.n + az.n);
       if (fmin_(ax.over, fmin_(ay.over, az.over)) < (double)J.max_time) {
         ++visits;
@@ -620,7 +647,9 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
       dda_init<M>(J.eye, r, s);
       int px = s.vx, py = s.vy, pz = s.vz;
       int entry = slot_of(J.cube, ls, px, py, pz);
-      while (dda_step<M>(s, r.L)) {
+      double L = r.L;
+      if (M == kDown) fp64_round_down_begin(s, L);
+      while (dda_step<M, false, M == kDown>(s, L)) {
         // miss on (px, py, pz)
         if (entry < 0) {
           ++errors;
@@ -659,6 +688,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
         py = s.vy;
         pz = s.vz;
       }
+      if (M == kDown) fp64_round_nearest();
     }
     __syncthreads();
     // flush the item's counts and leave the table empty for the next one

@@ -115,7 +115,11 @@ NM_HD void dda_init(const double* eye, const RayDir& r, Dda& s) {
 //          ny is, else x (+0 and -0 compare equal here as in the rule);
 //   else:  x only if strictly smallest, else y if y < z, else z, with t
 //          and the increment selected by bit masks.
-template <int M, bool kMin = false>
+//
+// kHwDown: the caller has set the wave's FP64 rounding mode to toward -inf,
+// so the round-down sum is the hardware add (one instruction instead of the
+// emulated add_down_pos).
+template <int M, bool kMin = false, bool kHwDown = false>
 NM_HD bool dda_step(Dda& s, double L) {
   const double nx = s.nx, ny = s.ny, nz = s.nz;
   const double ddx = s.dx, ddy = s.dy, ddz = s.dz;
@@ -139,7 +143,7 @@ NM_HD bool dda_step(Dda& s, double L) {
   }
   // t >= 0 and dt > 0 (next times start at 0 + a non-negative product and
   // grow by |1/d|), so the round-down sum needs no sign or zero cases
-  const double nt = M == kDown ? add_down_pos(t, dt) : t + dt;
+  const double nt = (M == kDown && !kHwDown) ? add_down_pos(t, dt) : t + dt;
   s.nx = bx ? nt : nx;
   s.ny = by ? nt : ny;
   s.nz = bz ? nt : nz;
