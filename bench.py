@@ -471,6 +471,27 @@ def main():
                      for k, (v, n) in fst.items()},
     }
     fmap.close()
+    # the lower-memory variants (PrecisionType 1-3): FP16 sums and count,
+    # voxel centre with an int count, voxel centre with an 8-bit count
+    modes = {}
+    for name, filt in (("fp16_centroid", nm.FILTER_VOXEL_HALF),
+                       ("centre_int_count", nm.FILTER_VOXEL_COUNT),
+                       ("centre_8bit_count", nm.FILTER_VOXEL_COUNT8)):
+        mparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width,
+                           cam.height, filt) for p, q, _ in frames[:nf]]
+        mmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
+        for k in range(2):
+            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+        torch.cuda.synchronize()
+        fa.record(stream)
+        for k in range(nf):
+            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+        fb.record(stream)
+        torch.cuda.synchronize()
+        modes[name] = {"ms_per_frame": round(fa.elapsed_time(fb) / nf, 4),
+                       "rays_last_frame": ctx.stats()["rays"]}
+        mmap.close()
+    filtered["precision_modes"] = modes
 
     # roofline of the dominant kernel stage: its duration over the timed
     # steps; the other stages from the untimed split
