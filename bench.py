@@ -80,7 +80,7 @@ def bench_batch(nm, W, torch, steps, rounding):
     """32 agents (uneven image sizes, ranges, voxel sizes; own maps): one
     graph-captured batched launch per pass, against the same frames as 32
     plain launches one after another, on the same stream."""
-    _, agents = W.cfg4_agents(nm)
+    _, agents = W.agents_32(nm)
     clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
     frames = [W.agent_frame(nm, a, rounding) for a in agents]
     P = max(c.shape[0] for c in clouds)
@@ -152,14 +152,14 @@ def bench_batch(nm, W, torch, steps, rounding):
 
 def bench_sim(nm, W, torch, steps):
     """Sensor-simulation cast (generateCloud): 64 poses of 848x480 at 90x60
-    deg, 15 m range, through the 0.1 m occupancy grid of the config-1 map
+    deg, 15 m range, through the 0.1 m occupancy grid of the box-room map
     thresholded at log-odds > 0; each pass casts all 64 poses in one call
     (which returns the hit count, so it synchronises)."""
-    o, v = W.cfg1_map_leaves(nm)
+    o, v = W.box_room_map_leaves(nm)
     grid = nm.SimGrid(o, v, 0.0)
     w, h, res = 848, 480, 0.1
     sensors = []
-    for pos, quat in W.cfg3_poses(nm, 64):
+    for pos, quat in W.sim_poses(nm, 64):
         fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, res, 15.0, 0.85, -0.4,
                                 0)
         sensors.append(nm.sim_sensor(pos, list(fr.rotation), res, w, h, 90.0,
@@ -176,7 +176,7 @@ def bench_sim(nm, W, torch, steps):
     rays = len(sensors) * w * h
     return {
         "workload": "64 poses x 848x480 (90x60 deg, 15 m) through the 0.1 m "
-                    "grid of the config-1 map, log-odds > 0",
+                    "grid of the box-room map, log-odds > 0",
         "rays": rays,
         "returns": n,
         "ms_per_pass": ms,
@@ -188,13 +188,13 @@ def bench_sim(nm, W, torch, steps):
 
 def bench_soft(nm, W, torch, steps):
     """Forward + adjoint of the differentiable soft march for 16 cameras of
-    320x240 (8 m, 160 samples at 0.05 m) over the config-1 world mapped at
+    320x240 (8 m, 160 samples at 0.05 m) over the box-room world mapped at
     0.05 m: one launch each for all cameras, against one launch pair per
     camera; plus 20 Gauss-Newton pose-refinement steps."""
     import numpy as np
     from nanomap_amd import soft
     res = 0.05
-    _, cam, pos, quat, clouds = W.cfg5(nm)
+    _, cam, pos, quat, clouds = W.soft_walk(nm)
     ctx = nm.Context(cam.width * cam.height, 8.0 / res + 1)
     m = nm.Map(res, *W.CLAMP, leaf_capacity=1 << 19)
     frames = []
@@ -276,7 +276,7 @@ def bench_soft(nm, W, torch, steps):
     r_err = torch.rad2deg(torch.arccos(cosang.clamp(-1, 1)))
     return {
         "workload": "16 cameras 320x240 (87x58 deg), 8 m, 160 samples at "
-                    "0.05 m, quadratic B-spline over the config-1 world "
+                    "0.05 m, quadratic B-spline over the box-room world "
                     "mapped at 0.05 m (%dx%dx%d grid); forward + adjoint "
                     "pose gradients" % grid.shape,
         "rays": rays,
@@ -355,7 +355,7 @@ def main():
                                                 max(args.sim_steps, 1))}))
         return
     t_setup = time.perf_counter()
-    scene, cam, frames = W.cfg2_frames(nm, n_frames=args.frames)
+    scene, cam, frames = W.hall_walk_frames(nm, n_frames=args.frames)
     P = cam.width * cam.height
     params = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width, cam.height)
               for p, q, _ in frames]

@@ -8,7 +8,7 @@
 // fesetround(), exactly as the reference executes it on the host.
 //
 // Parity status of each function (what pins it) is listed in the header of
-// tests/test_oracle.py and in DESIGN.md.
+// tests/test_oracle.py.
 
 #include "nm_oracle.h"
 

@@ -101,15 +101,15 @@ def test_frame_from_pose_closed_forms(nm):
 
 def test_scene_is_deterministic_and_plausible(nm):
     from nanomap_amd import workloads
-    _, cam, frames = workloads.cfg1(nm, seed=1)
-    _, _, frames2 = workloads.cfg1(nm, seed=1)
+    _, cam, frames = workloads.box_room_frame(nm, seed=1)
+    _, _, frames2 = workloads.box_room_frame(nm, seed=1)
     a, b = frames[0][2], frames2[0][2]
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     z = a[:, 2]
     assert np.all(z > 0) and np.all(z <= 5.0)
     # a good share of pixels sees a surface inside the 5 m cap
     assert np.mean(z < 5.0) > 0.1
-    _, _, frames3 = workloads.cfg1(nm, seed=2)
+    _, _, frames3 = workloads.box_room_frame(nm, seed=2)
     assert not np.array_equal(a, frames3[0][2])
 
 

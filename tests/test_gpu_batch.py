@@ -1,7 +1,7 @@
 """GPU parity of the batched path (include/nanomap_amd_batch.h) against
 separate plain launches of the same frames.
 
-Config 4: 32 agents with uneven image sizes, ranges and voxel sizes in one
+32 agents with uneven image sizes, ranges and voxel sizes in one
 100x100x10 m world, each agent with its own map.  A batch casts all agents'
 frames in one graph-captured launch sequence; the plain path (itself
 bit-exact against the oracle, test_gpu_cast.py) processes the same frames
@@ -48,15 +48,15 @@ def _plain(nm, ctx, agents, clouds, maps, rounding):
 
 
 @pytest.fixture(scope="module")
-def cfg4(gpu):
+def agents(gpu):
     nm = gpu
-    scene, a0 = W.cfg4_agents(nm, frame_index=0)
-    _, a1 = W.cfg4_agents(nm, scene=scene, frame_index=1)
+    scene, a0 = W.agents_32(nm, frame_index=0)
+    _, a1 = W.agents_32(nm, scene=scene, frame_index=1)
     return nm, a0, a1
 
 
-def test_cfg4_batch_bit_exact_vs_plain(cfg4):
-    nm, a0, a1 = cfg4
+def test_agents_32_batch_bit_exact_vs_plain(agents):
+    nm, a0, a1 = agents
     P = max(a["cloud"].shape[0] for a in a0)
     T = max(a["range"] / a["res"] for a in a0) + 1
     ctx = nm.Context(P, T)
@@ -90,8 +90,8 @@ def _subset(agents, n=8):
 
 
 @pytest.mark.parametrize("rounding", [RD, RN])
-def test_every_tuning_and_graph_mode_agree(cfg4, rounding):
-    nm, a0, _ = cfg4
+def test_every_tuning_and_graph_mode_agree(agents, rounding):
+    nm, a0, _ = agents
     agents = _subset(a0, 10)
     clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
     P = max(c.shape[0] for c in clouds)
@@ -118,11 +118,11 @@ def test_every_tuning_and_graph_mode_agree(cfg4, rounding):
                     m.close()
 
 
-def test_graph_replays_with_eager_work_between(cfg4):
+def test_graph_replays_with_eager_work_between(agents):
     """The captured graph is replayed for new frames of the same shape, with
     unrelated eager work and host synchronizations between replays; every
     replay must match the plain path."""
-    nm, a0, a1 = cfg4
+    nm, a0, a1 = agents
     agents0, agents1 = _subset(a0, 6), None
     ids = [a0.index(a) for a in agents0]
     agents1 = [a1[k] for k in ids]
@@ -146,8 +146,8 @@ def test_graph_replays_with_eager_work_between(cfg4):
             _assert_same_map(bm, pm, k)
 
 
-def test_autotuner_picks_a_valid_tuning(cfg4):
-    nm, a0, _ = cfg4
+def test_autotuner_picks_a_valid_tuning(agents):
+    nm, a0, _ = agents
     agents = _subset(a0, 8)
     clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
     T = max(a["range"] / a["res"] for a in agents) + 1
@@ -170,8 +170,8 @@ def test_autotuner_picks_a_valid_tuning(cfg4):
         _assert_same_map(bm, pm, k)
 
 
-def test_batch_rejects_bad_input(cfg4):
-    nm, a0, _ = cfg4
+def test_batch_rejects_bad_input(agents):
+    nm, a0, _ = agents
     a = _subset(a0, 2)
     clouds = [torch.from_numpy(x["cloud"]).cuda() for x in a]
     batch = nm.Batch(2, 10 ** 6, 400.0, 1 << 16)
@@ -194,10 +194,10 @@ def test_batch_rejects_bad_input(cfg4):
         batch.stats()
 
 
-def test_batch_with_per_agent_log_odds(cfg4):
+def test_batch_with_per_agent_log_odds(agents):
     """Agents with different hit/miss log-odds in one batch: each job's
     table of miss-only sums and its heavy-voxel sums use its own values."""
-    nm, a0, _ = cfg4
+    nm, a0, _ = agents
     agents = _subset(a0, 10)
     clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
     P = max(c.shape[0] for c in clouds)

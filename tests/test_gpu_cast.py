@@ -58,9 +58,9 @@ def _run_frame(nm, ctx, gmap, omap, frame, cloud, check_counts=True):
 
 
 @pytest.mark.parametrize("rounding", [RD, RN])
-def test_cfg1_counts_and_map_bit_exact(gpu, rounding):
+def test_box_room_counts_and_map_bit_exact(gpu, rounding):
     nm = gpu
-    _, cam, frames = W.cfg1(nm)
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, rounding, cam.width, cam.height)
     ctx = nm.Context(cam.width * cam.height, f.max_time)
@@ -77,9 +77,9 @@ def test_cfg1_counts_and_map_bit_exact(gpu, rounding):
     _check_maps(gmap, omap)
 
 
-def test_cfg2_ten_frames_bit_exact(gpu):
+def test_hall_ten_frames_bit_exact(gpu):
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=10)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=10)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -91,11 +91,11 @@ def test_cfg2_ten_frames_bit_exact(gpu):
 
 
 @pytest.mark.timeout(900)
-def test_cfg2_hundred_frames_bit_exact(gpu):
-    """100-frame 640x480 sequence at 0.05 m; every
+def test_hall_hundred_frames_bit_exact(gpu):
+    """The 100-frame 640x480 hall sequence at 0.05 m; every
     frame's counts and the final map bit-exact versus the CPU path."""
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=100)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=100)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -143,7 +143,7 @@ def test_edge_cases(gpu):
 def test_stride_16_and_device_pointer(gpu):
     import torch
     nm = gpu
-    _, cam, frames = W.cfg1(nm)
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, RD)
     padded = np.concatenate([cloud, np.full((len(cloud), 1), 7.0, np.float32)], 1)
@@ -162,7 +162,7 @@ def test_stride_16_and_device_pointer(gpu):
 
 def test_run_twice_is_identical(gpu):
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=1)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=1)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.05, 10.0, RD)
     ctx = nm.Context(len(cloud), f.max_time)
@@ -182,7 +182,7 @@ def test_unchanged_leaves_are_not_inserted(gpu):
     the leaves it reserved in the pool stay holes, invisible to leaf_count
     and download, and later frames over the same leaves still match."""
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=2)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -201,7 +201,7 @@ def test_unchanged_leaves_are_not_inserted(gpu):
 @pytest.mark.parametrize("rounding", [RD, RN])
 def test_voxel_filter_bit_exact(gpu, rounding):
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=2)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -228,7 +228,7 @@ def test_voxel_filter_precision_modes_bit_exact(gpu, mode):
     nm = gpu
     filt = {"half": nm.FILTER_VOXEL_HALF, "count": nm.FILTER_VOXEL_COUNT,
             "count8": nm.FILTER_VOXEL_COUNT8}[mode]
-    _, cam, frames = W.cfg2_frames(nm, n_frames=2)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=2)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -269,7 +269,7 @@ def test_voxel_filter_precision_modes_bit_exact(gpu, mode):
 
 def test_overflow_is_reported_not_fatal(gpu):
     nm = gpu
-    _, cam, frames = W.cfg1(nm)
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, RD)
     ctx = nm.Context(len(cloud), f.max_time, tile_capacity=4)
@@ -362,7 +362,7 @@ def test_log_odds_and_rounding_change_between_frames(gpu):
     change from frame to frame: the context's table of miss-only sums must
     follow the frame (it is rebuilt only when they change)."""
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=5)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=5)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
@@ -380,7 +380,7 @@ def test_stage_timing_all_and_masked(gpu):
     """Stage timing: every stage of a frame timed once per frame, or only
     the stages asked for; results unchanged either way."""
     nm = gpu
-    _, cam, frames = W.cfg1(nm)
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, RD, cam.width, cam.height)
     ctx = nm.Context(cam.width * cam.height, f.max_time)

@@ -1,10 +1,10 @@
 """GPU parity of the sensor-simulation cast (generateCloud) against the
 oracle, and the simulated clouds fed back through the mapping path.
 
-Config 3: 64 poses, 848x480 at 90x60 deg, 15 m range, through the 0.1 m
-occupancy grid of config 1's final map thresholded at log-odds > 0.
+64 poses, 848x480 at 90x60 deg, 15 m range, through the 0.1 m occupancy
+grid of the box room's final map thresholded at log-odds > 0.
 First-hit ijk and the FP32 hit distance must equal the oracle's bit for
-bit (BASELINE asks for ijk bit-exact and distance within 1e-5 m).
+bit (the bar is ijk bit-exact and distance within 1e-5 m).
 """
 import numpy as np
 import pytest
@@ -15,8 +15,8 @@ from nanomap_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_cfg1_leaves(nm):
-    _, cam, frames = W.cfg1(nm)
+def _oracle_box_room_leaves(nm):
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, nm.ROUND_DOWN, cam.width, cam.height)
     om = oracle.Map(*W.CLAMP)
@@ -35,17 +35,17 @@ def _sensors(nm, poses, res, w, h, hf, vf, rmin, rmax):
     return out
 
 
-def test_cfg3_sim_cast_bit_exact(gpu):
+def test_sim_cast_64_poses_bit_exact(gpu):
     nm = gpu
-    o, v = _oracle_cfg1_leaves(nm)
-    # the device map of config 1 equals the oracle's (test_gpu_cast); build
+    o, v = _oracle_box_room_leaves(nm)
+    # the device map of the box room equals the oracle's (test_gpu_cast); build
     # the device grid from the device map and the oracle grid from its own
-    gm_leaves = W.cfg1_map_leaves(nm)
+    gm_leaves = W.box_room_map_leaves(nm)
     assert np.array_equal(gm_leaves[0], o)
     gsim = nm.SimGrid(gm_leaves[0], gm_leaves[1], 0.0)
     osim = oracle.SimGrid(o, v, 0.0)
     assert gsim.info()["active"] == osim.active_count()
-    poses = W.cfg3_poses(nm, 64)
+    poses = W.sim_poses(nm, 64)
     sens = _sensors(nm, poses, 0.1, 848, 480, 90.0, 60.0, 0.1, 15.0)
     print("sim cast: 64 x 848x480", flush=True)
     ijk, dist, xyz, n = gsim.cast([s for _, _, s in sens])
@@ -68,7 +68,7 @@ def test_sim_on_dense_map_and_feedback(gpu):
     """A denser grid (three 640x480 frames at 0.05 m) and the simulated
     clouds fed back through nm_cast/nm_update_map, bit-exact end to end."""
     nm = gpu
-    _, cam, frames = W.cfg2_frames(nm, n_frames=3)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=3)
     om = oracle.Map(*W.CLAMP)
     for pos, quat, cloud in frames:
         f = W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN)

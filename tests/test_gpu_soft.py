@@ -1,6 +1,6 @@
 """GPU checks of the differentiable soft march (include/nanomap_amd_soft.h).
 
-Config 5: the config-1 world mapped at 0.05 m, 16 cameras of 320x240 with
+The box-room world mapped at 0.05 m, 16 cameras of 320x240 with
 an 8 m range (160 samples).  Checked:
   - the FP64 kernels against the FP64 oracle (forward and adjoint);
   - the FP32 forward against the FP64 one;
@@ -25,9 +25,9 @@ RES = 0.05
 
 
 @pytest.fixture(scope="module")
-def cfg5(gpu):
+def soft_walk(gpu):
     nm = gpu
-    _, cam, pos, quat, clouds = W.cfg5(nm)
+    _, cam, pos, quat, clouds = W.soft_walk(nm)
     ctx = nm.Context(cam.width * cam.height, 8.0 / RES + 1)
     m = nm.Map(RES, *W.CLAMP, leaf_capacity=1 << 19)
     frames = []
@@ -37,7 +37,7 @@ def cfg5(gpu):
         frames.append(f)
     o, v, _ = m.download()
     grid, lo = W.dense_grid(o, v)
-    print("cfg5 grid", grid.shape, "leaves", len(o), flush=True)
+    print("soft_walk grid", grid.shape, "leaves", len(o), flush=True)
     sel = list(range(0, 48, 3))[:16]
     R = torch.tensor(np.stack([np.array(list(frames[k].rotation),
                                         np.float64).reshape(3, 3) for k in sel]))
@@ -48,11 +48,11 @@ def cfg5(gpu):
     return nm, march, g32, R.cuda(), p.cuda(), grid, lo
 
 
-def _small(cfg5, w=16, h=12, ns=60):
+def _small(soft_walk, w=16, h=12, ns=60):
     """A cropped grid around camera 0 and a small image (oracle-sized).  The
     oracle gets the FP32 values of the voxel size and step that the kernel
     parameters hold."""
-    _, march, g32, R, p, grid, lo = cfg5
+    _, march, g32, R, p, grid, lo = soft_walk
     c = np.floor(p[0].cpu().numpy() / RES).astype(int) - lo
     a = np.maximum(c - 40, 0)
     b = np.minimum(c + 40, grid.shape)
@@ -67,8 +67,8 @@ def _small(cfg5, w=16, h=12, ns=60):
     return m, sub, cams, sc
 
 
-def test_fp64_kernels_match_oracle(cfg5):
-    m, sub, cams, sc = _small(cfg5)
+def test_fp64_kernels_match_oracle(soft_walk):
+    m, sub, cams, sc = _small(soft_walk)
     g64 = torch.from_numpy(sub.astype(np.float64)).cuda()
     d64 = m.forward64(g64, cams)
     od = sc.forward(cams.cpu().numpy())
@@ -86,8 +86,8 @@ def test_fp64_kernels_match_oracle(cfg5):
     assert np.abs(d32.cpu().numpy() - od).max() < 1e-4
 
 
-def test_batched_launch_equals_per_camera(cfg5):
-    _, march, g32, R, p, _, _ = cfg5
+def test_batched_launch_equals_per_camera(soft_walk):
+    _, march, g32, R, p, _, _ = soft_walk
     cams = soft.cameras(R, p)
     d = march.forward(g32, cams)
     for k in (0, 7, 15):
@@ -96,8 +96,8 @@ def test_batched_launch_equals_per_camera(cfg5):
     assert torch.isfinite(d).all() and float(d.min()) > 0
 
 
-def test_pose_gradient_matches_fp64_central_differences(cfg5):
-    _, march, g32, R, p, grid, _ = cfg5
+def test_pose_gradient_matches_fp64_central_differences(soft_walk):
+    _, march, g32, R, p, grid, _ = soft_walk
     g64 = g32.double()
     cams = soft.cameras(R, p)
     depth, logt = march.forward(g32, cams, with_logt=True)
@@ -121,8 +121,8 @@ def test_pose_gradient_matches_fp64_central_differences(cfg5):
     assert float(err.max()) <= 1e-4
 
 
-def test_logodds_gradient_matches_fp64_central_differences(cfg5):
-    _, march, g32, R, p, _, _ = cfg5
+def test_logodds_gradient_matches_fp64_central_differences(soft_walk):
+    _, march, g32, R, p, _, _ = soft_walk
     cams = soft.cameras(R[:4], p[:4])
     depth, logt = march.forward(g32, cams, with_logt=True)
     c = torch.randn(depth.shape, generator=torch.Generator().manual_seed(6)).cuda()
@@ -139,8 +139,8 @@ def test_logodds_gradient_matches_fp64_central_differences(cfg5):
         assert abs(float(flat[i]) - float(fd)) <= 1e-4 * abs(float(fd)) + 1e-7, (i, float(flat[i]), float(fd))
 
 
-def test_autograd_function_at_nonzero_perturbation(cfg5):
-    _, march, g32, R, p, _, _ = cfg5
+def test_autograd_function_at_nonzero_perturbation(soft_walk):
+    _, march, g32, R, p, _, _ = soft_walk
     R4, p4 = R[:4], p[:4]
     xi = torch.tensor([[0.02, -0.01, 0.015, 0.01, -0.02, 0.03]] * 4,
                       dtype=torch.float64, device="cuda", requires_grad=True)
@@ -172,8 +172,8 @@ def test_autograd_function_at_nonzero_perturbation(cfg5):
     assert float((gl.grad - raw).norm()) <= 1e-5 * float(raw.norm())
 
 
-def test_pose_refinement_recovers_perturbed_poses(cfg5):
-    _, march, g32, R, p, _, _ = cfg5
+def test_pose_refinement_recovers_perturbed_poses(soft_walk):
+    _, march, g32, R, p, _, _ = soft_walk
     target = march.forward(g32, soft.cameras(R, p))
     gen = torch.Generator().manual_seed(11)
     dt = (torch.rand((16, 3), generator=gen, dtype=torch.float64) * 2 - 1) * 0.1

@@ -16,10 +16,10 @@ import numpy as np
 import oracle
 
 
-def _cfg1_oracle_map():
+def _box_room_oracle_map():
     import nanomap_amd as nm
     from nanomap_amd import workloads as W
-    _, cam, frames = W.cfg1(nm)
+    _, cam, frames = W.box_room_frame(nm)
     pos, quat, cloud = frames[0]
     f = W.frame(nm, pos, quat, 0.1, 5.0, 1, cam.width, cam.height)
     om = oracle.Map(*W.CLAMP)
@@ -49,14 +49,14 @@ def _rays(pos, R, Wd, Hd, hf, vf, res):
 def test_hierarchical_matches_dim1_and_analytic(oracle_lib):
     import nanomap_amd as nm
     from nanomap_amd import workloads as W
-    om, f = _cfg1_oracle_map()
+    om, f = _box_room_oracle_map()
     o, v, _ = om.export()
     g = oracle.SimGrid(o, v, 0.0)
     assert g.active_count() > 100
     agree = total = 0
     errs = []
     Wd, Hd = 424, 240
-    for pos, quat in W.cfg3_poses(nm, 6):
+    for pos, quat in W.sim_poses(nm, 6):
         fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.1, 15.0, 0.85, -0.4, 0)
         R = np.array(list(fr.rotation), np.float32)
         a_ijk, a_d, _, _ = g.cast(pos, R, 0.1, Wd, Hd, 90.0, 60.0, 0.1, 15.0, True)
@@ -118,7 +118,7 @@ def test_dense_grid_terminates(oracle_lib):
     coarsening after 8 steps at one t, so every ray terminates."""
     import nanomap_amd as nm
     from nanomap_amd import workloads as W
-    _, cam, frames = W.cfg2_frames(nm, n_frames=3)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=3)
     om = oracle.Map(*W.CLAMP)
     for pos, quat, cloud in frames:
         f = W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN)
