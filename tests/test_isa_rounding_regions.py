@@ -3,10 +3,11 @@
 a region every rounded FP64 operation must be one whose round-down result is
 wanted: the crossing-time adds, plus operations that are exact in any mode.
 A divide or square root there would come out wrong (their expansions assume
-round-to-nearest, and the compiler rewrites the mode bits around them), and a
-branch that enters or leaves the region without passing a mode write would
-run code in the wrong mode.  This test compiles nm_cast.hip for gfx950 and
-checks every region in the generated assembly.  (CPU only: hipcc
+round-to-nearest, and the compiler rewrites the mode bits around them), and
+code reachable both with and without the mode set would run in an unknown
+mode.  This test compiles nm_cast.hip for gfx950, builds each kernel's
+control-flow graph, propagates the mode from the kernel entry and checks
+every instruction against the mode it runs in.  (CPU only: hipcc
 cross-compiles without a GPU.)
 """
 import os
@@ -65,54 +66,98 @@ def _functions(lines):
     return out
 
 
-def _regions(body):
-    regions, start = [], None
-    for k, t in enumerate(body):
-        if BEGIN.search(t):
-            assert start is None, "nested mode write"
-            start = k
-        elif END.search(t):
-            assert start is not None, "mode restored without being set"
-            regions.append((start, k))
-            start = None
-    assert start is None, "mode set and never restored"
-    return regions
+def _blocks(body):
+    """Basic blocks of a kernel body: (label, instructions, successors)."""
+    blocks, cur = [], ["<entry>", [], []]
+    blocks.append(cur)
+    for t in body:
+        if t.endswith(":"):
+            cur = [t[:-1], [], []]
+            blocks.append(cur)
+            continue
+        cur[1].append(t)
+        op = t.split()[0]
+        if op == "s_branch" or op.startswith("s_cbranch") or op == "s_endpgm":
+            cur = ["<after %d>" % len(blocks), [], []]
+            blocks.append(cur)
+    index = {b[0]: k for k, b in enumerate(blocks)}
+    for k, b in enumerate(blocks):
+        last = b[1][-1].split() if b[1] else [""]
+        op = last[0]
+        if op == "s_branch":
+            b[2] = [index[last[-1]]]
+        elif op.startswith("s_cbranch"):
+            b[2] = [index[last[-1]]] + ([k + 1] if k + 1 < len(blocks) else [])
+        elif op == "s_endpgm":
+            b[2] = []
+        else:
+            b[2] = [k + 1] if k + 1 < len(blocks) else []
+    return blocks
+
+
+def _mode_states(blocks):
+    """FP64 rounding mode at each instruction, by forward dataflow from the
+    kernel entry (round to nearest): 'rn', 'rd' or 'both'."""
+    def join(a, b):
+        return a if b is None or a == b else (b if a is None else "both")
+
+    def transfer(state, insns):
+        out = []
+        for t in insns:
+            out.append(state)
+            if BEGIN.search(t):
+                state = "rd"
+            elif END.search(t):
+                state = "rn"
+        return state, out
+
+    entry = [None] * len(blocks)
+    entry[0] = "rn"
+    work = [0]
+    while work:
+        k = work.pop()
+        st, _ = transfer(entry[k], blocks[k][1])
+        for n in blocks[k][2]:
+            j = join(entry[n], st)
+            if j != entry[n]:
+                entry[n] = j
+                work.append(n)
+    return [transfer(entry[k], b[1])[1] if entry[k] is not None
+            else [None] * len(b[1]) for k, b in enumerate(blocks)]
 
 
 def test_round_down_regions_hold_only_intended_operations(cast_asm):
     funcs = _functions(cast_asm)
     seen = {}
     for name, body in funcs:
-        regs = _regions(body)
-        if not regs:
+        if not any(BEGIN.search(t) for t in body):
             continue
         kind = "accumulate" if "k_accumulate" in name else (
             "discover" if "k_discover" in name else name)
-        seen[kind] = seen.get(kind, 0) + len(regs)
-        labels = {t[:-1]: k for k, t in enumerate(body) if t.endswith(":")}
-        for a, b in regs:
-            adds = 0
-            for k in range(a + 1, b):
-                t = body[k]
-                op = t.split()[0] if t else ""
-                assert not FORBIDDEN.match(op), (name, t)
-                if FP_OP.match(op):
-                    assert ALLOWED_FP.match(op), (name, t)
-                adds += op == "v_add_f64"
-                if op.startswith("s_cbranch") or op == "s_branch":
-                    tgt = t.split()[-1]
-                    assert a < labels.get(tgt, -1) < b, (
-                        "branch out of region", name, t)
-            assert adds >= 1, (name, "region without the crossing add")
-            # nothing outside jumps into the region
-            inside = {lab for lab, k in labels.items() if a < k < b}
-            for k, t in enumerate(body):
-                if a <= k <= b:
-                    continue
-                op = t.split()[0] if t else ""
-                if op.startswith("s_cbranch") or op == "s_branch":
-                    assert t.split()[-1] not in inside, (
-                        "branch into region", name, t)
+        seen[kind] = seen.get(kind, 0) + 1
+        blocks = _blocks(body)
+        states = _mode_states(blocks)
+        rd_adds = 0
+        for b, st in zip(blocks, states):
+            for t, m in zip(b[1], st):
+                op = t.split()[0]
+                if m == "both":
+                    # reached both with and without the mode set
+                    assert not FP_OP.match(op) and not FORBIDDEN.match(op), (
+                        "mode-ambiguous", name, b[0], t)
+                    assert not (BEGIN.search(t) or END.search(t)), (
+                        "mode-ambiguous mode write", name, b[0], t)
+                elif m == "rd":
+                    assert not FORBIDDEN.match(op) or END.search(t), (
+                        name, b[0], t)
+                    assert not op.startswith("s_swappc") and not \
+                        op.startswith("s_setpc"), ("call in region", name, t)
+                    if FP_OP.match(op):
+                        assert ALLOWED_FP.match(op), (name, b[0], t)
+                    rd_adds += op == "v_add_f64"
+                elif m == "rn":
+                    assert not END.search(t), ("restore outside region", name, t)
+        assert rd_adds >= 1, (name, "no crossing add in round-down mode")
     # both loops use the hardware mode in their round-down instantiations
     # (one k_discover, one k_accumulate per LDS table size)
     assert seen.get("discover", 0) >= 1 and seen.get("accumulate", 0) >= 2, seen
