@@ -71,8 +71,9 @@ int nm_batch_submit(nm_batch* batch, const nm_agent_frame* frames, int n,
 int nm_batch_stats(nm_batch* batch, nm_frame_stats* out, int n);
 
 /* Autotuner: times the cast of `frames` (without touching their maps)
- * under every candidate tuning, `reps` runs each, keeps the fastest and
- * returns it with its time.  Synchronizes. */
+ * under every candidate tuning, `reps` runs each, re-times the four
+ * fastest over longer interleaved runs, keeps the fastest and returns it
+ * with its time.  Synchronizes. */
 int nm_batch_tune(nm_batch* batch, const nm_agent_frame* frames, int n,
                   int reps, void* stream, nm_batch_tuning* best,
                   float* best_ms);

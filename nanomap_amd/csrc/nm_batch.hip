@@ -536,31 +536,54 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BH(hipEventCreate(&e0));
   BH(hipEventCreate(&e1));
-  float best_t = -1.0f;
-  nm::Tuning best_c = saved;
-  for (const nm::Tuning& t : cand) {
+  // ms per batch of tuning t over `runs` batches after one warm-up, or < 0
+  auto time_one = [&](const nm::Tuning& t, int runs) -> float {
     b->tuning = t;
     st = build_layout(b, f, n);
-    if (st != NM_OK) break;
+    if (st != NM_OK) return -1.0f;
     Staging& s = b->st[0];
     fill_staging(b, s, f, n, false);
     st = enqueue(b, s, q, true);  // warm-up
-    if (st != NM_OK) break;
-    if (hipEventRecord(e0, q) != hipSuccess) break;
-    for (int r = 0; r < reps && st == NM_OK; ++r) st = enqueue(b, s, q, true);
-    if (st != NM_OK) break;
+    if (st != NM_OK) return -1.0f;
+    if (hipEventRecord(e0, q) != hipSuccess) {
+      st = bfail(NM_EHIP, "nm_batch_tune: timing failed");
+      return -1.0f;
+    }
+    for (int r = 0; r < runs && st == NM_OK; ++r) st = enqueue(b, s, q, true);
+    if (st != NM_OK) return -1.0f;
     float ms = 0;
     if (hipEventRecord(e1, q) != hipSuccess ||
         hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
       st = bfail(NM_EHIP, "nm_batch_tune: timing failed");
-      break;
+      return -1.0f;
     }
-    ms /= reps;
-    if (best_t < 0 || ms < best_t) {
-      best_t = ms;
-      best_c = t;
-    }
+    return ms / runs;
+  };
+  // sweep every candidate, then re-time the four fastest in two
+  // interleaved rounds of longer runs, so that run-to-run noise of the
+  // short sweep does not decide between near-equal candidates
+  std::vector<std::pair<float, int>> swept;
+  for (size_t k = 0; k < cand.size() && st == NM_OK; ++k) {
+    const float ms = time_one(cand[k], reps);
+    if (st == NM_OK) swept.push_back({ms, (int)k});
+  }
+  float best_t = -1.0f;
+  nm::Tuning best_c = saved;
+  if (st == NM_OK && !swept.empty()) {
+    std::sort(swept.begin(), swept.end());
+    const size_t top = std::min<size_t>(4, swept.size());
+    std::vector<float> total(top, 0.0f);
+    const int runs = std::max(2 * reps, 4);
+    for (int round = 0; round < 2 && st == NM_OK; ++round)
+      for (size_t k = 0; k < top && st == NM_OK; ++k)
+        total[k] += time_one(cand[swept[k].second], runs);
+    if (st == NM_OK)
+      for (size_t k = 0; k < top; ++k)
+        if (best_t < 0 || total[k] / 2 < best_t) {
+          best_t = total[k] / 2;
+          best_c = cand[swept[k].second];
+        }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
