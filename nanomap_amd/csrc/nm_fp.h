@@ -4,9 +4,13 @@
 // The reference executes its host path with FE_DOWNWARD left set by
 // SensorData::updatePose (include/nanomap/sensor/SensorData.h:42), so the
 // hit/miss traversal and the FP32 log-odds sums round toward -inf.  On
-// gfx950 the MODE register could select that rounding, but the compiler's
+// gfx950 the MODE register can select that rounding, but the compiler's
 // mode-register pass resets the FP64 rounding bits around its own div/sqrt
-// sequences without restoring them.  So round-down is emulated exactly:
+// sequences without restoring them.  So outside the two per-step DDA loops
+// round-down is emulated exactly here (those loops hold no divide or square
+// root; nm_cast.hip sets the wave's FP64 mode to toward -inf around them
+// and back to nearest after, and dda_step's kHwDown form then uses the
+// hardware add):
 //   add: TwoSum error term; result stepped down one ulp if the error < 0;
 //   mul: FMA residual a*b - p; stepped down if < 0;
 //   div: FMA residual a - q*b of the correctly rounded quotient;
@@ -302,39 +306,40 @@ NM_HD float fold(float v, float a, uint32_t k) {
       dq = rint_(q);
     }
     const double d = dq * ulp;  // exact
-    if (d == 0.0) return v;     // every further step leaves v unchanged
-    const bool away = ((double)v > 0) == (d > 0);
-    const double ad = abs_(d);
-    int64_t n;
-    if (away) {
-      // step i valid while |v| + i|d| + |a| < 2^(e+1)
-      const double X = hi - aa - av;  // exact in double for |v| < 2^29
-      if (X <= 0) {
-        n = 0;
-      } else {
-        // quotient from an approximate reciprocal, then made exact
-        int64_t i = (int64_t)floor_(X * approx_rcp(ad));
-        if (i < 0) i = 0;
-        while (i > 0 && (double)i * ad >= X) --i;
-        while ((double)(i + 1) * ad < X) ++i;
-        n = ((double)i * ad < X) ? i + 1 : 0;
-      }
-    } else {
-      // step i valid while |v| - i|d| - |a| >= 2^e
-      const double Y = av - aa - lo;
-      if (Y < 0) {
-        n = 0;
-      } else {
-        int64_t i = (int64_t)floor_(Y * approx_rcp(ad));
-        if (i < 0) i = 0;
-        while (i > 0 && (double)i * ad > Y) --i;
-        while ((double)(i + 1) * ad <= Y) ++i;
-        n = i + 1;
-      }
+    const bool away = ((double)v > 0) == (a > 0.0f);
+    // Bounds in whole ulps, all exact integers in double: m = |d|/u,
+    // qa = |a|/u (exact scaling), and |v| - 2^e, 2^(e+1) - |v| (multiples
+    // of u below 2^24 u).  Step i (from |v| -+ i|d|) keeps the exact sum in
+    // the binade iff
+    //   away:      i m + qa <  (2^(e+1) - |v|)/u  <=>  i m <= that - floor(qa) - 1
+    //   toward 0:  i m + qa <= (|v| - 2^e)/u      <=>  i m <= that - ceil(qa)
+    // (i m is an integer).  Working in ulps keeps |a| exactly: a difference
+    // such as |v| - |a| - 2^e in double drops an |a| below the double ulp of
+    // |v| and would let the last step land on 2^e while the exact sum is
+    // just below it.
+    const double m = abs_(dq);
+    const double qa = abs_(q);
+    const double lim = away ? (hi - av) * inv_ulp - floor_(qa) - 1.0
+                            : (av - lo) * inv_ulp + floor_(-qa);
+    int64_t n = 0;
+    if (lim >= 0.0) {
+      // the first step stays in the binade: with d == 0 it leaves v as it
+      // is, and so does every further step
+      if (m == 0.0) return v;
+      // floor(lim / m) from an approximate reciprocal, then made exact
+      int64_t i = (int64_t)floor_(lim * approx_rcp(m));
+      if (i < 0) i = 0;
+      while (i > 0 && (double)i * m > lim) --i;
+      while ((double)(i + 1) * m <= lim) ++i;
+      n = i + 1;
     }
     if (n <= 0) {
-      v = add<M, kFinite>(v, a);
+      // one rounded step across the binade edge; a step that leaves v as
+      // it is repeats forever
+      const float nv = add<M, kFinite>(v, a);
       --k;
+      if (bits_of(nv) == bits_of(v)) return v;
+      v = nv;
       continue;
     }
     if ((uint64_t)n > k) n = (int64_t)k;

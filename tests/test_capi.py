@@ -55,6 +55,62 @@ def test_fold_host_matches_oracle_sequential_sums(nm, oracle_lib):
             assert g == o, (mode, v, a, k)
 
 
+def _pow2_neighbours():
+    out = []
+    for e in (-20, -3, 0, 1, 5, 23, 40):
+        p = np.float32(2.0 ** e)
+        for f in (p, np.nextafter(p, np.float32(np.inf)),
+                  np.nextafter(np.nextafter(p, np.float32(np.inf)),
+                               np.float32(np.inf)),
+                  np.float32(p * np.float32(1.0 + 3 * 2.0 ** -23)),
+                  np.nextafter(p, np.float32(-np.inf))):
+            out += [float(f), -float(f)]
+    return out
+
+
+def test_fold_host_tiny_increments_at_binade_edges(nm, oracle_lib):
+    """Increments far below the double ulp of v, with v on or next to a
+    power of two: a jump must stop where the exact sum first leaves the
+    binade, also when the step would land exactly on 2^e."""
+    rng = np.random.default_rng(22)
+    tiny = [1e-20, 2.0 ** -60, 3e-30, 1e-12, 2.0 ** -40 * 3, 7e-9]
+    cases = [(1.0, -1e-20, 30), (float(np.float32(1.0 + 3 * 2.0 ** -23)),
+                                 -2.0 ** -60, 100)]
+    for v in _pow2_neighbours():
+        for t in tiny:
+            for sgn in (-1.0, 1.0):
+                for k in (25, 26, 31, 64, 200, 5000):
+                    cases.append((v, float(np.float32(sgn * t)), k))
+    # increments a little below/above one ulp of v, and random mantissas
+    for _ in range(400):
+        e = int(rng.integers(-10, 30))
+        v = float(np.float32(rng.choice([-1, 1]) * 2.0 ** e *
+                             rng.choice([1.0, 1.0 + 2.0 ** -23,
+                                         rng.uniform(1, 2)])))
+        ulp = 2.0 ** (e - 23)
+        a = float(np.float32(rng.choice([-1, 1]) * ulp *
+                             rng.choice([0.25, 0.5, 0.75, 1.0, 1.5,
+                                         rng.uniform(0, 3)])))
+        cases.append((v, a, int(rng.integers(25, 3000))))
+    for mode in (0, 1):
+        for v, a, k in cases:
+            g = np.float32(nm.fold(v, a, k, mode)).view(np.uint32)
+            o = np.float32(oracle.fold(v, a, k, mode)).view(np.uint32)
+            assert g == o, (mode, v, a, k, hex(int(g)), hex(int(o)))
+    # the reported case, against the exact rational round-down sum
+    from fractions import Fraction
+    acc = np.float32(1.0)
+    for _ in range(30):
+        x = Fraction(float(acc)) + Fraction(float(np.float32(-1e-20)))
+        f = np.float32(float(x))
+        while Fraction(float(f)) > x:
+            f = np.nextafter(f, np.float32(-np.inf))
+        acc = f
+    assert np.float32(nm.fold(1.0, -1e-20, 30, 1)) == acc
+    assert np.float32(acc).view(np.uint32) == np.float32(
+        float.fromhex("0x1.ffffc4p-1")).view(np.uint32)
+
+
 def test_fold_host_overflow_and_non_finite(nm, oracle_lib):
     big = float(np.finfo(np.float32).max)
     # sums that overflow: round-down keeps FLT_MAX going up, reaches -inf
