@@ -88,6 +88,17 @@ typedef struct nm_frame {
   int rounding;         /* NM_ROUND_*                                       */
   int filter;           /* NM_FILTER_*                                      */
   int width, height;    /* cloud layout (pcl_width, pcl_height)             */
+  /* Voxel filter only (NM_FILTER_VOXEL*; the unfiltered cast follows the
+   * reference's CPU path, which has neither check).  Sensor-frame bounds
+   * in metres: a point with a coordinate below box_min or above box_max
+   * is dropped (filterCloud.cu:100-111); the reference's bounds are the
+   * frustum's worldMin/worldMax (FrustumData::calculateSensorBounds,
+   * include/nanomap/sensor/FrustumData.h:50-83), see nm_frustum_box.
+   * All six zero: no bound.  max_range: a point is binned only if
+   * 0 < |p| < max_range (metres, the sensor's maxRange, filterCloud.cu:112);
+   * 0 takes max_time * grid_res. */
+  float box_min[3], box_max[3];
+  float max_range;
 } nm_frame;
 
 /* Counters of the most recent frame of a context. */
@@ -135,6 +146,47 @@ int nm_map_download(nm_map* map, int32_t* origins, float* values,
 int nm_map_device_arrays(nm_map* map, void** values, void** masks,
                          void** origins, void** hash_keys, void** hash_vals,
                          int64_t* hash_capacity);
+
+/* Pool capacity in leaves. */
+int nm_map_capacity(nm_map* map, int64_t* leaf_capacity);
+/* The parameters the map was created with (any pointer may be NULL). */
+int nm_map_params(nm_map* map, float* grid_res, float* clamp_min,
+                  float* clamp_max, float* hit_thr, float* miss_thr);
+/* Grows the map to hold at least leaf_capacity leaves (pool and hash),
+ * keeping every leaf, value and mask.  Synchronous; a no-op when the map
+ * already holds that many.  The reference's OpenVDB grid grows per leaf
+ * (OccupancyMap.h:132); this pool grows on request or by auto-grow. */
+int nm_map_reserve(nm_map* map, int64_t leaf_capacity);
+/* Auto-grow (off by default): nm_update_map first waits for the cast and,
+ * when the map's free leaves are fewer than the leaves the cast touched,
+ * grows the map (at least doubling) before folding the cast in, so no
+ * update is lost to a full map.  Off, the update stays asynchronous and a
+ * full map is reported as overflow in the frame statistics. */
+int nm_map_set_auto_grow(nm_map* map, int enable);
+
+/* Batched point queries on the device: for n voxels (ijk, 3 int32 each)
+ * the log-odds value (0 where the leaf is absent: the grid background) and
+ * the active state (1/0).  Replaces reads through map::Map::occupiedGrid()
+ * / occAccessor() (include/nanomap/map/OccupancyMap.h:54,60).
+ * on_device = 1: ijk, values and active are device pointers and the call
+ * is asynchronous on `stream`; 0: host arrays, staged through the map's
+ * query buffer, and the call returns when the results are in place. */
+int nm_map_query(nm_map* map, const int32_t* ijk, int64_t n, float* values,
+                 uint8_t* active, int on_device, void* stream);
+
+/* Snapshot and restore.  nm_map_save writes the map to `path`: a 64-byte
+ * header ("NMAPLEAF", format version 1, leaf count, voxel size, clamps,
+ * thresholds), then per leaf its origin (3 int32), 512 FP32 log-odds
+ * (voxel z + 8y + 64x) and 16 active-mask words, little-endian, leaves in
+ * nm_map_download order.  nm_map_load creates a map from such a file on
+ * `device` with room for max(leaf_capacity, leaves in the file) leaves, or
+ * with leaf_capacity 0 for max(2 x those leaves, 2^18); the restored map
+ * continues exactly where the saved one stood.  The
+ * reference never saves its map; it only loads .vdb grids for simulation,
+ * SimManager.h:171-213. */
+int nm_map_save(nm_map* map, const char* path);
+int nm_map_load(int device, const char* path, int64_t leaf_capacity,
+                nm_map** out);
 
 /* ---- plain frame path --------------------------------------------------
  * nm_cast: filterCloud + frustumCastCloud (filterCloud.cu:476,

@@ -60,11 +60,24 @@ int nm_scene_walk(const nm_scene* s, uint64_t seed, int n, float step_m,
                   float step_deg, float max_pitch_deg, float clearance,
                   float* pos_out, float* quat_out);
 
-/* Per-frame host set-up in the given rounding mode (see above). */
+/* Per-frame host set-up in the given rounding mode (see above); also sets
+ * out->max_range = max_range and a zero (unbounded) filter box. */
 int nm_frame_from_pose(const float pos[3], const float quat[4],
                        const float F[9], float grid_res, float max_range,
                        float logodds_hit, float logodds_miss, int rounding,
                        nm_frame* out);
+
+/* Sensor-frame bounds of a frustum sensor, the voxel filter's point box
+ * (nm_frame.box_min/box_max): FrustumData::calculateSensorBounds
+ * (include/nanomap/sensor/FrustumData.h:50-83) -- aspect = width/height,
+ * half height h = range * tan(vfov/2), half width = aspect * h (the
+ * horizontal FOV is derived from the aspect, FrustumData.h:24), at
+ * min_range and max_range, plus the origin; box = per-axis min and max of
+ * those nine points.  Evaluated in `rounding` (the reference constructs
+ * its sensors with FE_DOWNWARD left set by an earlier one). */
+int nm_frustum_box(int width, int height, float vfov_deg, float min_range,
+                   float max_range, int rounding, float box_min[3],
+                   float box_max[3]);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ from ._capi import (FILTER_NONE, FILTER_VOXEL, FILTER_VOXEL_COUNT,
 
 __all__ = ["Context", "Map", "Batch", "Scene", "Camera", "Frame", "FrameStats",
            "SimGrid", "SimSensor", "sim_sensor",
-           "frame_from_pose", "fold", "NanoMapError", "ROUND_DOWN",
+           "frame_from_pose", "frustum_box", "fold", "NanoMapError", "ROUND_DOWN",
            "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "FILTER_VOXEL_HALF",
            "FILTER_VOXEL_COUNT", "FILTER_VOXEL_COUNT8", "DEFAULT_F"]
 
@@ -35,10 +35,24 @@ def fold(v, a, k, rounding):
     return lib().nm_fold_host(float(v), float(a), int(k), int(rounding))
 
 
+def frustum_box(width, height, vfov_deg, min_range, max_range,
+                rounding=ROUND_DOWN):
+    """(box_min, box_max): the voxel filter's sensor-frame point bounds of a
+    frustum sensor (FrustumData::calculateSensorBounds)."""
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    check(lib().nm_frustum_box(int(width), int(height), float(vfov_deg),
+                               float(min_range), float(max_range),
+                               int(rounding), _ptr(lo, ctypes.c_float),
+                               _ptr(hi, ctypes.c_float)))
+    return lo, hi
+
+
 def frame_from_pose(pos, quat, F=DEFAULT_F, grid_res=0.1, max_range=5.0,
                     logodds_hit=0.85, logodds_miss=-0.4, rounding=ROUND_DOWN,
-                    width=0, height=0, filter=FILTER_NONE):
-    """Per-frame host set-up (R, eye, max_time) as the reference does it."""
+                    width=0, height=0, filter=FILTER_NONE, box=None):
+    """Per-frame host set-up (R, eye, max_time) as the reference does it.
+    box: (box_min, box_max) of the voxel filter, e.g. from frustum_box()."""
     f = Frame()
     p = np.ascontiguousarray(pos, dtype=np.float32)
     q = np.ascontiguousarray(quat, dtype=np.float32)
@@ -49,6 +63,10 @@ def frame_from_pose(pos, quat, F=DEFAULT_F, grid_res=0.1, max_range=5.0,
                                    max_range, logodds_hit, logodds_miss,
                                    rounding, ctypes.byref(f)))
     f.width, f.height, f.filter = width, height, filter
+    if box is not None:
+        for a in range(3):
+            f.box_min[a] = float(box[0][a])
+            f.box_max[a] = float(box[1][a])
     return f
 
 
@@ -227,6 +245,56 @@ class Map:
         o, v, m = o[:L], v[:L], m[:L]
         order = np.lexsort((o[:, 2], o[:, 1], o[:, 0]))
         return o[order], v[order], m[order]
+
+    def capacity(self):
+        n = ctypes.c_int64()
+        check(lib().nm_map_capacity(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def reserve(self, leaf_capacity):
+        check(lib().nm_map_reserve(self.handle, int(leaf_capacity)))
+
+    def set_auto_grow(self, enable):
+        check(lib().nm_map_set_auto_grow(self.handle, int(bool(enable))))
+
+    def query(self, ijk, stream=None):
+        """(values [n], active [n] bool) of voxels ijk [n, 3]: host arrays,
+        or device tensors (int32 [n,3] -> returns device tensors)."""
+        if hasattr(ijk, "data_ptr") and ijk.is_cuda:
+            import torch
+            n = ijk.shape[0]
+            vals = torch.empty(n, dtype=torch.float32, device=ijk.device)
+            act = torch.empty(n, dtype=torch.uint8, device=ijk.device)
+            check(lib().nm_map_query(self.handle,
+                                     ctypes.c_void_p(ijk.data_ptr()), n,
+                                     ctypes.c_void_p(vals.data_ptr()),
+                                     ctypes.c_void_p(act.data_ptr()), 1,
+                                     stream))
+            return vals, act.bool()
+        a = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        vals = np.zeros(max(len(a), 1), np.float32)
+        act = np.zeros(max(len(a), 1), np.uint8)
+        check(lib().nm_map_query(self.handle, a.ctypes.data_as(ctypes.c_void_p),
+                                 len(a), vals.ctypes.data_as(ctypes.c_void_p),
+                                 act.ctypes.data_as(ctypes.c_void_p), 0,
+                                 stream))
+        return vals[:len(a)], act[:len(a)].astype(bool)
+
+    def save(self, path):
+        check(lib().nm_map_save(self.handle, str(path).encode()))
+
+    @classmethod
+    def load(cls, path, leaf_capacity=0, device=0):
+        h = ctypes.c_void_p()
+        check(lib().nm_map_load(device, str(path).encode(), int(leaf_capacity),
+                                ctypes.byref(h)))
+        m = cls.__new__(cls)
+        m.handle = h
+        res = ctypes.c_float()
+        check(lib().nm_map_params(h, ctypes.byref(res), None, None, None,
+                                  None))
+        m.grid_res = res.value
+        return m
 
 
 class Batch:

@@ -47,7 +47,8 @@ class Frame(ctypes.Structure):
                 ("logodds_hit", ctypes.c_float),
                 ("logodds_miss", ctypes.c_float), ("rounding", ctypes.c_int),
                 ("filter", ctypes.c_int), ("width", ctypes.c_int),
-                ("height", ctypes.c_int)]
+                ("height", ctypes.c_int), ("box_min", ctypes.c_float * 3),
+                ("box_max", ctypes.c_float * 3), ("max_range", ctypes.c_float)]
 
     def copy(self):
         f = Frame()
@@ -138,6 +139,14 @@ SIGNATURES = {
     "nm_map_device_arrays": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P),
                                   ctypes.POINTER(_P), ctypes.POINTER(_P),
                                   ctypes.POINTER(_P), _PI64]),
+    "nm_map_capacity": (_I, [_P, _PI64]),
+    "nm_map_params": (_I, [_P, _PF, _PF, _PF, _PF, _PF]),
+    "nm_map_reserve": (_I, [_P, ctypes.c_int64]),
+    "nm_map_set_auto_grow": (_I, [_P, _I]),
+    "nm_map_query": (_I, [_P, _P, ctypes.c_int64, _P, _P, _I, _P]),
+    "nm_map_save": (_I, [_P, ctypes.c_char_p]),
+    "nm_map_load": (_I, [_I, ctypes.c_char_p, ctypes.c_int64,
+                         ctypes.POINTER(_P)]),
     "nm_cast": (_I, [_P, ctypes.POINTER(Frame), _P, _I, _I, _I, _P]),
     "nm_update_map": (_I, [_P, _P, _P]),
     "nm_frame_submit": (_I, [_P, _P, ctypes.POINTER(Frame), _P, _I, _I, _I,
@@ -181,6 +190,7 @@ SIGNATURES = {
                            _PF]),
     "nm_frame_from_pose": (_I, [_PF, _PF, _PF, _F, _F, _F, _F, _I,
                                 ctypes.POINTER(Frame)]),
+    "nm_frustum_box": (_I, [_I, _I, _F, _F, _F, _I, _PF, _PF]),
     "nm_simgrid_from_leaves": (_I, [_I, _PI32, _PF, ctypes.c_int64, _F,
                                     ctypes.POINTER(_P)]),
     "nm_simgrid_from_map": (_I, [_P, _F, ctypes.POINTER(_P)]),

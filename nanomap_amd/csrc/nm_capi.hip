@@ -160,6 +160,25 @@ void free_ctx(nm_ctx* c) {
   delete c;
 }
 
+// The voxel filter's point bounds of a frame: its box unless all six
+// bounds are zero, and max_range (max_time * grid_res when 0).
+int filter_bounds(const nm_frame& f, nm::FilterBounds* fb) {
+  bool any = false;
+  for (int a = 0; a < 3; ++a) {
+    if (!std::isfinite(f.box_min[a]) || !std::isfinite(f.box_max[a]) ||
+        !(f.box_min[a] <= f.box_max[a]))
+      return fail(NM_EINVAL, "nm_cast: filter box must be finite with min <= max");
+    any = any || f.box_min[a] != 0.0f || f.box_max[a] != 0.0f;
+    fb->lo[a] = f.box_min[a];
+    fb->hi[a] = f.box_max[a];
+  }
+  fb->use_box = any ? 1 : 0;
+  if (!(f.max_range >= 0.0f) || !std::isfinite(f.max_range))
+    return fail(NM_EINVAL, "nm_cast: max_range must be finite and >= 0");
+  fb->max_range = f.max_range > 0.0f ? f.max_range : f.max_time * f.grid_res;
+  return NM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -285,7 +304,8 @@ int nm_map_destroy(nm_map* m) {
   (void)hipSetDevice(m->device);
   (void)hipDeviceSynchronize();
   void* ptrs[] = {m->v.hash_keys, m->v.hash_vals, m->v.values,
-                  m->v.masks,     m->v.origins,   m->v.leaf_count};
+                  m->v.masks,     m->v.origins,   m->v.leaf_count,
+                  m->query_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -394,6 +414,11 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(frame->eye[a]) || fabs(frame->eye[a]) > 1e8)
       return fail(NM_EINVAL, "nm_cast: eye must be finite");
+  nm::FilterBounds fb{};
+  if (frame->filter != NM_FILTER_NONE) {
+    int st = filter_bounds(*frame, &fb);
+    if (st != NM_OK) return st;
+  }
   if (ctx->pending)
     return fail(NM_ESTATE, "nm_cast: the previous cast was not folded into a map (nm_update_map)");
   NM_HIP(hipSetDevice(ctx->device));
@@ -467,8 +492,9 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     float origin[3];
     for (int a = 0; a < 3; ++a)
       origin[a] = (float)(frame->eye[a] - floor(frame->eye[a]));
-    NM_HIP(nm::launch_voxel_filter(p, origin, frame->filter - NM_FILTER_VOXEL,
-                                   ctx->b, s));
+    NM_HIP(nm::launch_voxel_filter(p, origin, fb,
+                                   frame->filter - NM_FILTER_VOXEL, ctx->b,
+                                   s));
   }
   {
     StageTimer t(ctx, NM_STAGE_DISCOVER, s);
@@ -508,6 +534,23 @@ int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
     return fail(NM_EINVAL, "nm_update_map: map and context on different devices");
   NM_HIP(hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
+  if (map->auto_grow) {
+    // wait for the cast and the map's last work, then make room for every
+    // leaf the cast touched (an upper bound on the leaves it adds)
+    NM_HIP(hipStreamSynchronize(ctx->stream));
+    NM_HIP(hipStreamSynchronize(map->stream));
+    nm::FrameCounters c;
+    NM_HIP(hipMemcpy(&c, ctx->b.counters, sizeof(c), hipMemcpyDeviceToHost));
+    int32_t n[2] = {0, 0};
+    NM_HIP(hipMemcpy(n, map->v.leaf_count, sizeof(n), hipMemcpyDeviceToHost));
+    const int64_t slots = std::min<int64_t>(n[0], map->v.leaf_cap);
+    const int64_t need = slots + std::min<int64_t>(c.n_leaves, ctx->tile_cap);
+    if (need > map->v.leaf_cap) {
+      const int st = nm::map_reserve(
+          map, std::max<int64_t>(need, 2 * (int64_t)map->v.leaf_cap));
+      if (st != NM_OK) return st;
+    }
+  }
   {
     StageTimer t(ctx, NM_STAGE_UPDATE, s);
     // the table of miss runs is rebuilt only when the miss log-odds or the

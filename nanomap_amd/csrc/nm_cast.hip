@@ -714,23 +714,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
   if ((threadIdx.x & 63) == 0 && se) atomicAdd(&cnt->errors, (int)se);
 }
 
-// ---- map hash ----------------------------------------------------------
-__device__ __forceinline__ unsigned long long pack_leaf(int x, int y, int z) {
-  const unsigned long long m = (1ull << 21) - 1;
-  return (((unsigned long long)((x >> 3) + (1 << 20)) & m) << 42) |
-         (((unsigned long long)((y >> 3) + (1 << 20)) & m) << 21) |
-         ((unsigned long long)((z >> 3) + (1 << 20)) & m);
-}
-
-__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ull;
-  k ^= k >> 33;
-  return k;
-}
-
+// ---- map hash (pack_leaf, mix64: nm_internal.h) --------------------------
 // Leaf origin (index space) of entry c of job J's cube table.
 __device__ __forceinline__ void slot_origin(const Job& J, int c, int& ox,
                                             int& oy, int& oz) {

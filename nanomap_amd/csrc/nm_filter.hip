@@ -30,7 +30,12 @@ struct FilterKey {
   int offset;  // added to each bin coordinate
 };
 
+// Validity (filterCloud.cu:98-112): no NaN coordinate, inside the
+// sensor-frame box on every axis, 0 < |p| < maxRange; then the bin
+// floor(r/res + voxelOriginOffset) and the fractions r/res - floor(r/res)
+// of r = R p (filterCloud.cu:114-128).
 __device__ __forceinline__ void point_bin(const CastParams& p,
+                                          const FilterBounds& fb,
                                           const float* origin, int i,
                                           bool& valid, int b[3], float f[3]) {
   const float* pt =
@@ -39,9 +44,11 @@ __device__ __forceinline__ void point_bin(const CastParams& p,
   valid = false;
   if (__builtin_isnan(px) || __builtin_isnan(py) || __builtin_isnan(pz))
     return;
-  const float max_range = p.max_time * p.res;
+  if (fb.use_box && (px < fb.lo[0] || px > fb.hi[0] || py < fb.lo[1] ||
+                     py > fb.hi[1] || pz < fb.lo[2] || pz > fb.hi[2]))
+    return;
   const float norm = __builtin_sqrtf(px * px + py * py + pz * pz);
-  if (!(norm > 0.0f && norm < max_range)) return;
+  if (!(norm > 0.0f && norm < fb.max_range)) return;
   const float r[3] = {p.R[0] * px + p.R[1] * py + p.R[2] * pz,
                       p.R[3] * px + p.R[4] * py + p.R[5] * pz,
                       p.R[6] * px + p.R[7] * py + p.R[8] * pz};
@@ -55,8 +62,8 @@ __device__ __forceinline__ void point_bin(const CastParams& p,
 }
 
 __global__ __launch_bounds__(256) void k_filter_keys(
-    CastParams p, float o0, float o1, float o2, FilterKey fk,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    CastParams p, FilterBounds fb, float o0, float o1, float o2,
+    FilterKey fk, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
     FrameCounters* __restrict__ cnt) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p.n) return;
@@ -64,7 +71,7 @@ __global__ __launch_bounds__(256) void k_filter_keys(
   bool valid;
   int b[3];
   float f[3];
-  point_bin(p, origin, i, valid, b, f);
+  point_bin(p, fb, origin, i, valid, b, f);
   uint32_t key = kInvalid;
   if (valid) {
     const int lim = (1 << fk.bits) - 1;  // top value reserved
@@ -86,8 +93,9 @@ __global__ __launch_bounds__(256) void k_filter_keys(
 // precision: 0 FP32 centroid, 1 FP16 centroid, 2 voxel centre (int
 // count), 3 voxel centre (8-bit count).
 __global__ __launch_bounds__(256) void k_filter_bins(
-    CastParams p, float o0, float o1, float o2, FilterKey fk, int precision,
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    CastParams p, FilterBounds fb, float o0, float o1, float o2,
+    FilterKey fk, int precision, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ vals,
     const FrameCounters* __restrict__ cnt, float4* __restrict__ tmp,
     int32_t* __restrict__ flags) {
   const int nv = cnt->n_filter_points;
@@ -111,7 +119,7 @@ __global__ __launch_bounds__(256) void k_filter_bins(
           bool valid;
           int b[3];
           float f[3];
-          point_bin(p, origin, (int)vals[q], valid, b, f);
+          point_bin(p, fb, origin, (int)vals[q], valid, b, f);
           if (precision == 0) {
             sx += f[0];
             sy += f[1];
@@ -231,8 +239,8 @@ __global__ __launch_bounds__(256) void k_flag_emit(
 }  // namespace
 
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               int precision, const Workspace& b,
-                               hipStream_t s) {
+                               const FilterBounds& fb, int precision,
+                               const Workspace& b, hipStream_t s) {
   if (p.n <= 0) return hipSuccess;
   FilterKey fk;
   const int need = (int)__builtin_ceilf(p.max_time) + 3;
@@ -241,8 +249,8 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
   fk.offset = 1 << (fk.bits - 1);
   if (3 * fk.bits > 31) return hipErrorInvalidValue;
   const int grid = (p.n + 255) / 256;
-  k_filter_keys<<<grid, 256, 0, s>>>(p, origin[0], origin[1], origin[2], fk,
-                                     b.keys[0], b.vals[0], b.counters);
+  k_filter_keys<<<grid, 256, 0, s>>>(p, fb, origin[0], origin[1], origin[2],
+                                     fk, b.keys[0], b.vals[0], b.counters);
   uint32_t* keys[2] = {b.keys[0], b.keys[1]};
   uint32_t* vals[2] = {b.vals[0], b.vals[1]};
   int out = 0;
@@ -250,7 +258,7 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
                                   &out, s);
   if (e != hipSuccess) return e;
   k_filter_bins<<<min(grid, 4096), 256, 0, s>>>(
-      p, origin[0], origin[1], origin[2], fk, precision, b.keys[out],
+      p, fb, origin[0], origin[1], origin[2], fk, precision, b.keys[out],
       b.vals[out],
       b.counters, reinterpret_cast<float4*>(b.filter_tmp), b.flags);
   const int nblocks = (p.n + kScanChunk - 1) / kScanChunk;

@@ -59,6 +59,42 @@ struct MapView {
   float clamp_min, clamp_max, hit_thr, miss_thr;
 };
 
+// Map hash key of the leaf holding voxel (x, y, z): leaf coordinates
+// (x >> 3 ...) biased by 2^20, 21 bits each; and the hash of a key.
+__host__ __device__ __forceinline__ unsigned long long pack_leaf(int x, int y,
+                                                                  int z) {
+  const unsigned long long m = (1ull << 21) - 1;
+  return (((unsigned long long)((x >> 3) + (1 << 20)) & m) << 42) |
+         (((unsigned long long)((y >> 3) + (1 << 20)) & m) << 21) |
+         ((unsigned long long)((z >> 3) + (1 << 20)) & m);
+}
+
+__host__ __device__ __forceinline__ unsigned long long mix64(
+    unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// Pool index of the leaf holding voxel (x, y, z), or -1 (open addressing,
+// linear probing; an empty slot ends the probe).
+__device__ __forceinline__ int map_find(const MapView& m, int x, int y,
+                                        int z) {
+  const unsigned long long key = pack_leaf(x, y, z);
+  const unsigned long long mask = (unsigned long long)m.hash_cap - 1;
+  unsigned long long h = mix64(key) & mask;
+  for (int64_t probe = 0; probe < m.hash_cap; ++probe) {
+    const unsigned long long k = m.hash_keys[h];
+    if (k == key) return m.hash_vals[h];
+    if (k == ~0ull) return -1;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
 // Cube of candidate leaves around the eye: leaf coordinates
 // [cmin, cmin + side) per axis.  Every voxel a ray can reach lies inside,
 // because rays are shorter than max_time voxels.
@@ -245,13 +281,22 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
 int radix_max_blocks(int n);
 int radix_hist_words(int n);  // histogram buffer size for n keys
 
+// Point bounds of the voxel filter (filterCloud.cu:100-112): a point is
+// binned only if it has no NaN coordinate, lies inside [lo, hi] on every
+// axis (when use_box), and 0 < |p| < max_range.
+struct FilterBounds {
+  float lo[3], hi[3];
+  float max_range;
+  int use_box;
+};
+
 // Voxel filter: bins points by voxel, writes one ray per bin into
 // b.filter_rays (count -> counters->n_filter_rays).  origin: fractional
 // part of the eye, eye - floor(eye), as FP32 (the reference's
 // voxelOriginOffset, include/nanomap/sensor/SensorData.h:55).
 hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
-                               int precision, const Workspace& b,
-                               hipStream_t s);
+                               const FilterBounds& fb, int precision,
+                               const Workspace& b, hipStream_t s);
 
 void set_error(const std::string& msg);
 
@@ -289,6 +334,15 @@ struct nm_map {
   float grid_res = 0;
   nm::MapView v{};
   hipStream_t stream = nullptr;
+  bool auto_grow = false;
+  // staging of host-array point queries (nm_map_query), grown on demand
+  void* query_buf = nullptr;
+  int64_t query_cap = 0;  // points
 };
+
+namespace nm {
+// Grows m's pool and hash to at least `capacity` leaves (nm_map.hip).
+int map_reserve(nm_map* m, int64_t capacity);
+}  // namespace nm
 
 #endif  // NM_INTERNAL_H

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cfenv>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -331,6 +332,38 @@ int nm_frame_from_pose(const float pos[3], const float quat[4],
   out->logodds_miss = logodds_miss;
   out->rounding = rounding;
   out->filter = NM_FILTER_NONE;
+  out->max_range = max_range;
+  return NM_OK;
+}
+
+int nm_frustum_box(int width, int height, float vfov_deg, float min_range,
+                   float max_range, int rounding, float box_min[3],
+                   float box_max[3]) {
+  if (!box_min || !box_max || width <= 0 || height <= 0 ||
+      !(vfov_deg > 0.0f && vfov_deg < 180.0f) || !(min_range >= 0.0f) ||
+      !(max_range >= min_range) || !std::isfinite(max_range) ||
+      (rounding != NM_ROUND_NEAREST && rounding != NM_ROUND_DOWN))
+    return NM_EINVAL;
+  const int saved = fegetround();
+  fesetround(rounding == NM_ROUND_DOWN ? FE_DOWNWARD : FE_TONEAREST);
+  // FrustumData.h:24, 69-72: FP32 aspect; the half extents are double
+  // products rounded to float; the widths FP32 products.
+  volatile float aspect = (float)width / (float)height;
+  volatile double t = tan(vfov_deg * M_PI / 360);
+  volatile float hh = (float)(max_range * t);
+  volatile float hw = aspect * hh;
+  volatile float ch = (float)(min_range * t);
+  volatile float cw = aspect * ch;
+  fesetround(saved);
+  const float x[9] = {-cw, -cw, cw, cw, -hw, -hw, hw, hw, 0.0f};
+  const float y[9] = {-ch, ch, ch, -ch, -hh, hh, hh, -hh, 0.0f};
+  const float z[9] = {min_range, min_range, min_range, min_range,
+                      max_range, max_range, max_range, max_range, 0.0f};
+  const float* c[3] = {x, y, z};
+  for (int a = 0; a < 3; ++a) {
+    box_min[a] = *std::min_element(c[a], c[a] + 9);
+    box_max[a] = *std::max_element(c[a], c[a] + 9);
+  }
   return NM_OK;
 }
 

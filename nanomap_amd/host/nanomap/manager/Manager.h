@@ -8,7 +8,8 @@
 // here it is one nm_frame_submit into the device-resident map, with the
 // reference CPU path's semantics (bit-exact to it, including its
 // round-toward-minus-infinity arithmetic).  Like the reference call it
-// returns once the map is updated.
+// returns once the map is updated; a frame that overflows a capacity throws
+// (the map grows instead of overflowing: map::Map turns auto-grow on).
 #ifndef NANOMAP_AMD_HOST_MANAGER_H
 #define NANOMAP_AMD_HOST_MANAGER_H
 
@@ -59,17 +60,28 @@ class Manager {
                                NM_ROUND_DOWN, &f));
       f.width = pcl_width;
       f.height = pcl_height;
-      // FilterType 1 with PrecisionType 0-3 (filterCloud.cu:476-553).
-      const int prec = _config->precisionType();
-      f.filter = _config->filterType() == 1 && prec >= 0 && prec <= 3
-                     ? NM_FILTER_VOXEL + prec
-                     : NM_FILTER_NONE;
+      f.filter = filterFor(sh);
+      if (f.filter != NM_FILTER_NONE) {
+        // the voxel filter drops points outside the frustum's sensor-frame
+        // box (filterCloud.cu:100-111)
+        for (int a = 0; a < 3; ++a) {
+          f.box_min[a] = sh._worldMin[a];
+          f.box_max[a] = sh._worldMax[a];
+        }
+      }
       const int n = pcl_width * pcl_height;
       nm_ctx* ctx = context(*s, n, f.max_time);
       check(nm_frame_submit(ctx, map->handle(), &f, cloud, n, pcl_step, 0,
                             nullptr));
-      int64_t leaves = 0;
-      check(nm_map_leaf_count(map->handle(), &leaves));  // waits for the map
+      // waits for the update, as the reference call does; a frame that
+      // did not fit is an error, not a silently partial map
+      nm_frame_stats st;
+      check(nm_frame_stats_get(ctx, &st));
+      if (st.errors != 0 || st.overflow != 0)
+        throw std::runtime_error(
+            "insertPointCloud: frame incomplete (" +
+            std::to_string(st.overflow) + " capacity overflows, " +
+            std::to_string(st.errors) + " errors)");
       _seconds += std::chrono::duration<double>(
                       std::chrono::steady_clock::now() - t0).count();
       ++_frames;
@@ -88,6 +100,19 @@ class Manager {
 
   std::shared_ptr<nanomap::config::Config> config() { return _config; }
   int64_t frames() const { return _frames; }
+
+  // The cast of a sensor's clouds: frustum sensors with FilterType 1 take
+  // the voxel filter of their PrecisionType (filterCloud.cu:476-553);
+  // laser sensors and FilterType 0 cast every point, as the reference's
+  // CPU path does (Handler.cpp:247-249, 28-86).
+  int filterFor(const sensor::SharedParameters& sh) const {
+    if (sh._type != 0 || _config->filterType() != 1) return NM_FILTER_NONE;
+    const int prec = _config->precisionType();
+    if (prec < 0 || prec > 3)
+      throw std::runtime_error("FilterType 1 needs PrecisionType 0-3, got " +
+                               std::to_string(prec));
+    return NM_FILTER_VOXEL + prec;
+  }
 
  private:
   // One device workspace per sensor, sized for its largest cloud and range.

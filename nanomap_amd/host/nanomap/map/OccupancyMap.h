@@ -3,14 +3,17 @@
 // Mirrors include/nanomap/map/OccupancyMap.h:35-61: the constructor takes
 // the voxel size and the hit/miss probability thresholds and derives the
 // log-odds thresholds and the clamping values logit(0.97) / logit(0.12).
-// The grid lives in device memory (nm_map); leaves() copies it to the host
-// in place of the reference's OpenVDB grid accessor.
+// The grid lives in device memory (nm_map): getValue()/query() read voxels
+// through device point queries in place of the reference's OpenVDB
+// accessor, leaves() copies the whole grid to the host, save() and the
+// snapshot constructor store and restore it.
 #ifndef NANOMAP_AMD_HOST_MAP_H
 #define NANOMAP_AMD_HOST_MAP_H
 
 #include <cmath>
 #include <cstdint>
 #include <map>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -37,8 +40,21 @@ class Map {
     check(nm_map_create(device, gridRes, _emptyClampingThreshold,
                         _occupiedClampingThreshold, _logOddsHitThreshold,
                         _logOddsMissThreshold, leafCapacity, &_map));
+    // like the reference's OpenVDB grid, the map never runs out of leaves
+    check(nm_map_set_auto_grow(_map, 1));
+  }
+  // A map restored from an nm_map_save snapshot (voxel size, clamps and
+  // thresholds come from the file).
+  explicit Map(const std::string& snapshot, int device = 0) {
+    check(nm_map_load(device, snapshot.c_str(), 0, &_map));
+    check(nm_map_set_auto_grow(_map, 1));
+    check(nm_map_params(_map, &_gridRes, &_emptyClampingThreshold,
+                        &_occupiedClampingThreshold, &_logOddsHitThreshold,
+                        &_logOddsMissThreshold));
   }
   ~Map() { nm_map_destroy(_map); }
+
+  void save(const std::string& path) const { check(nm_map_save(_map, path.c_str())); }
   Map(const Map&) = delete;
   Map& operator=(const Map&) = delete;
 
@@ -65,24 +81,23 @@ class Map {
     if (n) check(nm_map_download(_map, origins.data(), values.data(), masks.data()));
   }
 
-  // Value and activity of voxel (i, j, k) from a host copy (false if the
-  // voxel's leaf is not allocated: background 0, inactive).
+  // Value and activity of voxel (i, j, k), an accessor read
+  // (occAccessor()->probeValue, OccupancyMap.h:60): one device point query.
+  // A voxel of an absent leaf reads as the background, 0 and inactive.
+  // Returns the active state.
   bool getValue(int i, int j, int k, float& value, bool& active) const {
-    std::vector<int32_t> o;
-    std::vector<float> v;
-    std::vector<uint32_t> m;
-    leaves(o, v, m);
-    const int li = (k & 7) + ((j & 7) << 3) + ((i & 7) << 6);
-    for (size_t l = 0; l < o.size() / 3; ++l)
-      if (o[3 * l] == (i & ~7) && o[3 * l + 1] == (j & ~7) &&
-          o[3 * l + 2] == (k & ~7)) {
-        value = v[l * 512 + li];
-        active = (m[l * 16 + (li >> 5)] >> (li & 31)) & 1u;
-        return true;
-      }
-    value = 0.0f;
-    active = false;
-    return false;
+    const int32_t ijk[3] = {i, j, k};
+    uint8_t a = 0;
+    check(nm_map_query(_map, ijk, 1, &value, &a, 0, nullptr));
+    active = a != 0;
+    return active;
+  }
+
+  // Batched reads: values and active states (1/0) of n voxels (ijk: 3 ints
+  // each), from host arrays; device arrays go to nm_map_query directly.
+  void query(const int32_t* ijk, int64_t n, float* values,
+             uint8_t* active) const {
+    check(nm_map_query(_map, ijk, n, values, active, 0, nullptr));
   }
 
   nm_map* handle() { return _map; }

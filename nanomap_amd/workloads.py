@@ -47,9 +47,15 @@ def hall_walk_frames(nm, n_frames=100, seed=2, first=0):
     return scene, cam, out
 
 
-def frame(nm, pos, quat, res, rng, rounding, w=0, h=0, filt=0):
+def frame(nm, pos, quat, res, rng, rounding, w=0, h=0, filt=0, box=None):
     return nm.frame_from_pose(pos, quat, nm.DEFAULT_F, res, rng, HIT, MISS,
-                              rounding, w, h, filt)
+                              rounding, w, h, filt, box=box)
+
+
+def hall_box(nm, rounding):
+    """The voxel filter's sensor-frame box of the hall camera (640x480, 58
+    deg vertical FOV, 10 m; the half width follows the pixel aspect)."""
+    return nm.frustum_box(640, 480, 58.0, 0.0, 10.0, rounding)
 
 
 def sim_poses(nm, n_poses=64, seed=3):

@@ -40,7 +40,8 @@ SIGNATURES = {
     "or_dda_trace": (_I, [_PD, _PF, _F, _F, _I, _PI32, _I,
                           ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "or_fold": (_F, [_F, _F, ctypes.c_uint32, _I]),
-    "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _PF, _I, _I]),
+    "or_voxel_filter": (_I, [_P, _I, _I, _PF, _PD, _F, _F, _F, _PF, _PF, _I,
+                             _I]),
     "or_simgrid_build": (_P, [_PI32, _PF, ctypes.c_int64, _F]),
     "or_simgrid_destroy": (None, [_P]),
     "or_simgrid_active_count": (ctypes.c_int64, [_P]),
@@ -191,6 +192,20 @@ def fold(v, a, k, rounding):
     return lib().or_fold(float(v), float(a), int(k), int(rounding))
 
 
+def filter_bounds(frame):
+    """(max_range, box [6] or None) of a frame's voxel filter: the frame's
+    box_min/box_max unless all six are zero, and its max_range (metres;
+    max_time * grid_res when 0 or absent)."""
+    mr = float(getattr(frame, "max_range", 0.0) or 0.0)
+    if mr == 0.0:
+        mr = float(np.float32(np.float32(frame.max_time) *
+                              np.float32(frame.grid_res)))
+    lo = list(getattr(frame, "box_min", (0.0, 0.0, 0.0)))
+    hi = list(getattr(frame, "box_max", (0.0, 0.0, 0.0)))
+    box = np.array(lo + hi, np.float32)
+    return mr, (None if not np.any(box) else box)
+
+
 def voxel_filter(frame, cloud, max_rays=None, precision=None):
     """Rays of the voxel filter; precision 0-3 as PrecisionType (default:
     from frame.filter, NM_FILTER_VOXEL.. -> 0..3)."""
@@ -199,11 +214,13 @@ def voxel_filter(frame, cloud, max_rays=None, precision=None):
     a = np.ascontiguousarray(cloud, dtype=np.float32)
     a = a.reshape(a.shape[0], -1) if a.size else a.reshape(0, 3)
     R, eye = _frame_args(frame)
+    max_range, box = filter_bounds(frame)
     max_rays = max_rays or a.shape[0]
     out = np.zeros((max(max_rays, 1), 4), np.float32)
     n = lib().or_voxel_filter(a.ctypes.data_as(_P), a.shape[0], a.shape[1] * 4,
                               _p(R, ctypes.c_float), _p(eye, ctypes.c_double),
-                              frame.grid_res, frame.max_time,
+                              frame.grid_res, frame.max_time, max_range,
+                              None if box is None else _p(box, ctypes.c_float),
                               _p(out, ctypes.c_float), max_rays, precision)
     return out[:min(n, max_rays)]
 

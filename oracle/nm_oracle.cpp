@@ -429,7 +429,9 @@ float or_fold(float v0, float a, uint32_t k, int rounding) {
 // floor(r/res + voxelOriginOffset), r = R p; per bin the fractions
 // frac(r/res) are summed in ascending pixel order; each bin emits the ray
 // (mean + bin) normalised with length |mean + bin| if 0 < length < max_time.
-// Validity: no NaN coordinate and 0 < |p| < max range (max_time * res).
+// Validity (filterCloud.cu:98-112): no NaN coordinate, inside the
+// sensor-frame box [worldMin, worldMax] on every axis (when given), and
+// 0 < |p| < maxRange.
 // Bins are relative to floor(eye) and emitted in ascending
 // (x, y, z) order.
 //
@@ -454,15 +456,14 @@ static float half_rn(double x) {
 
 int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     const float R[9], const double eye[3], float res,
-                    float max_time, float* rays4, int max_rays,
-                    int precision) {
+                    float max_time, float max_range, const float* box,
+                    float* rays4, int max_rays, int precision) {
   RoundGuard guard(0);
   struct Bin {
     float sx, sy, sz;
     uint32_t n;
     float hx, hy, hz, hn;  // FP16 sums (precision 1)
   };
-  const float max_range = max_time * res;
   float origin[3];
   for (int a = 0; a < 3; ++a)
     origin[a] = (float)(eye[a] - std::floor(eye[a]));
@@ -483,6 +484,10 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
     std::memcpy(&py, p + 4, 4);
     std::memcpy(&pz, p + 8, 4);
     if (std::isnan(px) || std::isnan(py) || std::isnan(pz)) continue;
+    // filterCloud.cu:100-111: outside the sensor-frame box on any axis
+    if (box && (px < box[0] || px > box[3] || py < box[1] || py > box[4] ||
+                pz < box[2] || pz > box[5]))
+      continue;
     const float norm = std::sqrt(px * px + py * py + pz * pz);
     if (!(norm > 0.0f && norm < max_range)) continue;
     const float r[3] = {R[0] * px + R[1] * py + R[2] * pz,

@@ -75,13 +75,16 @@ int or_dda_trace(const double eye[3], const float d[3], float res,
 float or_fold(float v0, float a, uint32_t k, int rounding);
 
 /* ---- voxel filter (filterCloud.cu voxelFilter + rayFromFilter) ------- */
-/* Deterministic restatement: bins points by voxel, averages in pixel
- * order, emits one ray (dir xyz, length) per occupied bin in ascending key
- * order.  Returns the number of rays written (<= max_rays). */
+/* Deterministic restatement: drops points with a NaN coordinate, outside
+ * the sensor-frame box (box = min xyz, max xyz in metres; NULL = no box) or
+ * without 0 < |p| < max_range (metres), bins the rest by voxel, averages in
+ * pixel order, and emits one ray (dir xyz, length) per occupied bin in
+ * ascending key order.  Returns the number of rays (written up to
+ * max_rays). */
 int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     const float R[9], const double eye[3], float res,
-                    float max_time, float* rays4, int max_rays,
-                    int precision);
+                    float max_time, float max_range, const float* box,
+                    float* rays4, int max_rays, int precision);
 
 /* ---- sensor-simulation cast (generateCloud.cu:20-82, Sensor.h:156-227) - */
 typedef struct or_simgrid or_simgrid;

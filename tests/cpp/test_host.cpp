@@ -3,10 +3,12 @@
 //
 //   test_host parse <config>
 //       parse a config with nanomap::config::Config and print its values;
-//   test_host map <config> <sensor> <width> <height> <out> <frame>...
+//   test_host map <config> <sensor> <width> <height> <out>
+//                 [--capacity <leaves>] <frame>...
 //       each <frame> is "<cloud.bin>,px,py,pz,qw,qx,qy,qz": xyz float32
 //       points (width x height, stride 12), pose position and orientation;
-//       inserts the frames into one nanomap::map::Map with
+//       inserts the frames into one nanomap::map::Map (created with room
+//       for <leaves> leaves, default 2^18; it grows as needed) with
 //       Manager::insertPointCloud and writes the map's leaves to <out>
 //       (int64 count, then origins int32 [n,3], values float32 [n,512],
 //       masks uint32 [n,16]).
@@ -47,12 +49,17 @@ static int map(int argc, char** argv) {
   auto cfg = std::make_shared<nanomap::config::Config>(argv[2]);
   const std::string sensor = argv[3];
   const int w = std::atoi(argv[4]), h = std::atoi(argv[5]);
+  int first = 7, capacity = 1 << 18;
+  if (argc > 8 && std::string(argv[7]) == "--capacity") {
+    capacity = std::atoi(argv[8]);
+    first = 9;
+  }
   nanomap::manager::Manager manager(cfg);
-  auto m = std::make_shared<nanomap::map::Map>(cfg->mappingRes(),
-                                               cfg->probHitThres(),
-                                               cfg->probMissThres());
+  auto m = std::make_shared<nanomap::map::Map>(
+      cfg->mappingRes(), cfg->probHitThres(), cfg->probMissThres(), 0.0f,
+      0.0f, 0, capacity);
   std::vector<float> cloud((size_t)w * h * 3);
-  for (int a = 7; a < argc; ++a) {
+  for (int a = first; a < argc; ++a) {
     std::stringstream ss(argv[a]);
     std::string path, tok;
     std::getline(ss, path, ',');

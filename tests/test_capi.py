@@ -192,3 +192,36 @@ def test_context_needs_device(nm):
         pytest.skip("device present: covered by gpu tests")
     with pytest.raises(nm.NanoMapError):
         nm.Context(100, 50.0)
+
+
+def test_frustum_box_closed_form(nm):
+    # FrustumData::calculateSensorBounds: half height = range tan(vfov/2),
+    # half width = aspect * half height, z from 0 (the origin corner) to
+    # the range; the half extents are positive products, so round-down
+    # shrinks the box on both sides.
+    for w, h, vfov, rmin, rmax in ((640, 480, 58.0, 0.0, 10.0),
+                                   (160, 120, 45.0, 0.3, 5.0),
+                                   (848, 480, 60.0, 0.1, 15.0)):
+        lo_n, hi_n = nm.frustum_box(w, h, vfov, rmin, rmax, nm.ROUND_NEAREST)
+        lo_d, hi_d = nm.frustum_box(w, h, vfov, rmin, rmax, nm.ROUND_DOWN)
+        hh = rmax * np.tan(np.radians(vfov) / 2)
+        hw = w / h * hh
+        assert np.allclose(hi_n, [hw, hh, rmax], rtol=2e-7)
+        assert np.allclose(lo_n, [-hw, -hh, 0.0], rtol=2e-7)
+        assert np.all(hi_d <= hi_n) and np.all(lo_d >= lo_n)
+        assert np.all(np.abs(hi_d - hi_n) <= 2 * np.spacing(hi_n))
+    # 87x58 deg at 640x480: the box is the aspect-derived 72.9 deg, not 87
+    lo, hi = nm.frustum_box(640, 480, 58.0, 0.0, 10.0)
+    assert abs(np.degrees(2 * np.arctan(hi[0] / 10.0)) - 72.9) < 0.1
+    with pytest.raises(nm.NanoMapError):
+        nm.frustum_box(640, 480, 190.0, 0.0, 10.0)
+
+
+def test_map_load_rejects_bad_files_before_any_device_work(nm, tmp_path):
+    # the snapshot is parsed and checked on the host first
+    p = tmp_path / "junk.nmap"
+    p.write_bytes(b"NOTAMAP!" + bytes(100))
+    with pytest.raises(nm.NanoMapError):
+        nm.Map.load(p)
+    with pytest.raises(nm.NanoMapError):
+        nm.Map.load(tmp_path / "missing.nmap")

@@ -206,7 +206,8 @@ def test_voxel_filter_bit_exact(gpu, rounding):
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
     for pos, quat, cloud in frames:
-        f = W.frame(nm, pos, quat, 0.05, 10.0, rounding, filt=nm.FILTER_VOXEL)
+        f = W.frame(nm, pos, quat, 0.05, 10.0, rounding, filt=nm.FILTER_VOXEL,
+                    box=W.hall_box(nm, rounding))
         ctx.cast(f, cloud)
         g_rays = ctx.filter_rays()
         o_rays = oracle.voxel_filter(f, cloud)
@@ -233,7 +234,8 @@ def test_voxel_filter_precision_modes_bit_exact(gpu, mode):
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
     for pos, quat, cloud in frames:
-        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, filt=filt)
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, filt=filt,
+                    box=W.hall_box(nm, RD))
         ctx.cast(f, cloud)
         g_rays = ctx.filter_rays()
         o_rays = oracle.voxel_filter(f, cloud)
@@ -265,6 +267,60 @@ def test_voxel_filter_precision_modes_bit_exact(gpu, mode):
     assert np.array_equal(g_rays.view(np.uint32), o_rays.view(np.uint32))
     if mode == "count8":
         assert len(o_rays) == len(oracle.voxel_filter(f, cloud, precision=2)) - 1
+
+
+def _box_probes(cloud, lo, hi, n, rng):
+    """Copies of n valid cloud points moved onto each face of the box, one
+    float outside it and one float inside it."""
+    ok = ~np.isnan(cloud).any(1)
+    base = cloud[ok][rng.choice(int(ok.sum()), n, replace=False)]
+    out = []
+    for a in range(3):
+        for face, away in ((lo[a], -np.inf), (hi[a], np.inf)):
+            f32 = np.float32(face)
+            for v in (f32, np.nextafter(f32, np.float32(away)),
+                      np.nextafter(f32, np.float32(-away))):
+                p = base.copy()
+                p[:, a] = v
+                out.append(p)
+    return np.concatenate(out).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", ["voxel", "half", "count", "count8"])
+def test_voxel_filter_box_edges_bit_exact(gpu, mode):
+    # The frustum box of the 640x480, 58 deg, 10 m sensor (half width from
+    # the aspect: +-7.39 m at 10 m, while the 87 deg camera sees to +-9.5 m):
+    # the frame's own far-edge points fall outside it, and probe points on
+    # each face, one float outside and one inside, are kept or dropped as
+    # the oracle keeps or drops them, in every precision mode.
+    nm = gpu
+    filt = {"voxel": nm.FILTER_VOXEL, "half": nm.FILTER_VOXEL_HALF,
+            "count": nm.FILTER_VOXEL_COUNT,
+            "count8": nm.FILTER_VOXEL_COUNT8}[mode]
+    rng = np.random.default_rng(31)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=1)
+    pos, quat, cloud = frames[0]
+    lo, hi = W.hall_box(nm, RD)
+    cloud = np.concatenate([cloud, _box_probes(cloud, lo, hi, 400, rng)])
+    f = W.frame(nm, pos, quat, 0.05, 10.0, RD, filt=filt, box=(lo, hi))
+    unboxed = W.frame(nm, pos, quat, 0.05, 10.0, RD, filt=filt)
+    assert len(oracle.voxel_filter(f, cloud)) < \
+        len(oracle.voxel_filter(unboxed, cloud))
+    ctx = nm.Context(len(cloud), 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    print("box edges: points=%d filter=%d" % (len(cloud), filt), flush=True)
+    ctx.cast(f, cloud)
+    g_rays = ctx.filter_rays()
+    o_rays = oracle.voxel_filter(f, cloud)
+    assert g_rays.shape == o_rays.shape and len(o_rays) > 1000
+    assert np.array_equal(g_rays.view(np.uint32), o_rays.view(np.uint32))
+    temp = oracle.Temp()
+    temp.cast_rays(f, o_rays)
+    _check_counts(ctx, temp)
+    ctx.update(gmap)
+    omap.integrate(temp, RD)
+    _check_maps(gmap, omap)
 
 
 def test_overflow_is_reported_not_fatal(gpu):

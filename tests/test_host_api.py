@@ -79,7 +79,7 @@ FrameTransform:
 """
 
 
-def _write_config(tmp, res, filter_type, sensors, precision=0):
+def _write_config(tmp, res, filter_type, sensors, precision=0, update_type=0):
     paths = []
     for i, text in enumerate(sensors):
         p = os.path.join(tmp, "sensor%d.txt" % i)
@@ -90,8 +90,9 @@ def _write_config(tmp, res, filter_type, sensors, precision=0):
     with open(cfg, "w") as f:
         f.write("#config\nMappingRes:\n%s\nProbHitThres:\n0.5\nProbMissThres:\n"
                 "0.5\nGridRes:\n%s\nNodeEdge:\n8\nFrustumAllocationFactor:\n0.4\n"
-                "FilterType:\n%d\nUpdateType:\n0\nPrecisionType:\n%d\n"
-                "PublishSensor:\n1\n" % (res, res, filter_type, precision))
+                "FilterType:\n%d\nUpdateType:\n%d\nPrecisionType:\n%d\n"
+                "PublishSensor:\n1\n" % (res, res, filter_type, update_type,
+                                         precision))
         for p in paths:
             f.write("Sensor:\n%s\n" % p)
         f.write("#endconfig\n")
@@ -153,16 +154,24 @@ def _read_map(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("filter_type,precision",
-                         [(0, 0), (1, 0), (1, 1), (1, 2), (1, 3)])
+@pytest.mark.parametrize("filter_type,precision,update_type,capacity",
+                         [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0),
+                          (1, 2, 0, 0), (1, 3, 0, 0), (0, 0, 1, 0),
+                          (0, 0, 0, 16), (1, 0, 1, 8)])
 def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
-                                       precision):
-    # FilterType 1 with PrecisionType 0-3 selects the four voxel filters
+                                       precision, update_type, capacity):
+    # FilterType 1 with PrecisionType 0-3 selects the four voxel filters,
+    # which drop points outside the frustum's sensor-frame box.  UpdateType
+    # 1 (the reference's per-leaf BlockWorker fold, BlockWorker.cpp:95-176)
+    # adds, clamps and thresholds exactly as UpdateType 0, so its map is the
+    # same.  capacity > 0 starts the map with that many leaves: it grows
+    # as the frames need, and no update is lost.
     nm = gpu
     res, rmax, w, h = 0.1, 5.0, 160, 120
     cfg = _write_config(str(tmp_path), res, filter_type, [
         FRUSTUM.format(name="cam", id=1, w=w, h=h, vfov=45, rmax=rmax,
-                       ph=0.7, pm=0.4)], precision)
+                       ph=0.7, pm=0.4)], precision, update_type)
+    box = nm.frustum_box(w, h, 45.0, 0.1, rmax, nm.ROUND_DOWN)
     scene = nm.Scene(1, (20.0, 20.0, 5.0), 40, 0.3, 2.0)
     pos, quat = scene.walk(8, 3, 0.1, 2.0, 15.0, 0.3)
     cam = nm.camera(w, h, 60.0, 45.0, rmax, noise_k=0.002, dropout=0.05)
@@ -181,7 +190,7 @@ def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
         f = nm.frame_from_pose(p, q, nm.DEFAULT_F, res, rmax, float(_logit(0.7)),
                                float(_logit(0.4)), nm.ROUND_DOWN, w, h,
                                nm.FILTER_VOXEL + precision if filter_type == 1
-                               else nm.FILTER_NONE)
+                               else nm.FILTER_NONE, box=box)
         t = oracle.Temp()
         if filter_type == 1:
             t.cast_rays(f, oracle.voxel_filter(f, cloud, precision=precision))
@@ -190,9 +199,11 @@ def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
         omap.integrate(t, nm.ROUND_DOWN)
     out = os.path.join(str(tmp_path), "map.bin")
     print("host driver: 3 frames, filter", filter_type, "precision", precision,
-          flush=True)
-    r = subprocess.run([driver, "map", cfg, "cam", str(w), str(h), out] + args,
-                       capture_output=True, text=True, timeout=120)
+          "update", update_type, "capacity", capacity, flush=True)
+    extra = ["--capacity", str(capacity)] if capacity else []
+    r = subprocess.run([driver, "map", cfg, "cam", str(w), str(h), out] +
+                       extra + args, capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     go, gv, gm = _read_map(out)
     oo, ov, omk = omap.export()

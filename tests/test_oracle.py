@@ -18,7 +18,14 @@ oracle is pinned by what the mathematics fixes:
     survey (7 x -0.4f: 0xC0333334 nearest, 0xC0333335 down);
   temp accumulate + integrate (Handler.cpp:88-122): closed forms for one
     ray, repeated frames, clamp saturation, activity thresholds;
-  voxel filter restatement: an independent pure-numpy binning.
+  voxel filter: closed forms on exact inputs (integer eye: the endpoint
+    is the bin's centroid and lies inside it; one point per bin: that
+    point, or the point + 1 on axes where the origin offset carries; the
+    voxel-centre modes emit bin + 0.5 and the 8-bit count drops exactly
+    bins of 256 k points), brute-force bin counts on tiny clouds, the
+    frustum box faces at the 640x480 87x58 deg sensor (on the face kept,
+    one ulp outside dropped), the strict range cap, and FP16 sums against
+    numpy's float16.
 
 Parity unpinned (no check can reach them without the reference build):
 Eigen's quaternion rounding, OpenVDB's compiled worldToIndex, the
@@ -289,107 +296,223 @@ def test_duplicate_rays_sum_in_pixel_order(oracle_lib):
 
 
 # ------------------------------------------------------------ filter ----
+# The voxel filter (filterCloud.cu:84-133, 279-318) is pinned by what the
+# mathematics fixes, on inputs where every quantity is exact: an identity
+# or signed-permutation R (r = R p has no rounding), a power-of-two voxel
+# size (q = r / res is exact) and points on a 1/64 m grid, so bins and
+# centroids have closed forms to compare with.
 
-def _filter_numpy(cloud, R, eye, res, max_time, precision=0):
-    """Ascending-key rays; precision 1 sums FP16-rounded fractions and the
-    count in numpy's float16 (correctly rounded: float32 holds the exact
-    sum of two halves' 11-bit significands to 24 bits, so rounding twice
-    is innocuous), 2 and 3 give voxel centres, 3 with an 8-bit count."""
-    R = np.asarray(R, np.float32).reshape(3, 3)
-    origin = np.array([np.float32(e - math.floor(e)) for e in eye], np.float32)
+DEFAULT_F = [0.0, 0.0, 1.0, -1.0, 0.0, 0.0, 0.0, 1.0, 0.0]  # r = (z, -x, y)
+
+
+def _rays_endpoints(rays):
+    """Endpoints (dir * len) of filter rays, in voxel units (float64)."""
+    r = np.asarray(rays, np.float64)
+    return r[:, :3] * r[:, 3:4]
+
+
+def _clustered(rng, n_bins, per_bin, res, span=6):
+    """Points on a 1/64 m grid, per_bin(k) of them in each of n_bins
+    distinct voxels of size res (metres) around the origin."""
+    pts, seen = [], set()
+    while len(seen) < n_bins:
+        b = tuple(int(x) for x in rng.integers(-span, span, size=3))
+        if b in seen or b == (0, 0, 0):
+            continue
+        seen.add(b)
+        k = per_bin(len(seen))
+        cells = rng.integers(0, int(64 * res), size=(k, 3))
+        pts.append((np.asarray(b) * res + cells / 64.0).astype(np.float32))
+    return np.concatenate(pts)
+
+
+@pytest.mark.parametrize("rot", ["identity", "frame"])
+def test_voxel_filter_integer_eye_gives_bin_centroids(oracle_lib, rot):
+    # Integer eye: voxelOriginOffset = 0, so a point's bin is floor(q) and
+    # the emitted endpoint frac-mean + bin is the mean of the bin's q: the
+    # centroid, which lies inside the bin.  One ray per occupied bin, in
+    # ascending (x, y, z) bin order.
+    rng = np.random.default_rng(51)
+    res = 0.25
+    cloud = _clustered(rng, 40, lambda k: 1 + k % 7, res)
+    cloud = cloud[rng.permutation(len(cloud))]
+    f = _F([12.0, -5.0, 2.0], res=res, max_time=60.0)
+    f.rotation = [1, 0, 0, 0, 1, 0, 0, 0, 1] if rot == "identity" else DEFAULT_F
+    Rm = np.asarray(f.rotation, np.float64).reshape(3, 3)
+    rays = oracle.voxel_filter(f, cloud)
+    q = (cloud.astype(np.float64) @ Rm.T) / res  # exact
     bins = {}
-    max_range = np.float32(np.float32(max_time) * np.float32(res))
-    for p in cloud:
-        p = p.astype(np.float32)
-        if np.any(np.isnan(p)):
-            continue
-        nrm = np.sqrt(np.float32(np.float32(p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]))
-        if not (nrm > 0 and nrm < max_range):
-            continue
-        r = [np.float32(np.float32(R[i, 0] * p[0] + R[i, 1] * p[1]) + R[i, 2] * p[2]) for i in range(3)]
-        key, fr = [], []
+    for qi in q:
+        bins.setdefault(tuple(int(math.floor(x)) for x in qi), []).append(qi)
+    keys = sorted(bins)
+    # (R negates x for "frame": points on a voxel's lower x face then fall
+    # into the neighbouring bin, so there can be more than 40 bins)
+    assert len(rays) == len(keys) >= 40
+    ends = _rays_endpoints(rays)
+    for k, e in zip(keys, ends):
+        mean = [sum(Fraction(float(x[a])) for x in bins[k]) / len(bins[k])
+                for a in range(3)]
         for a in range(3):
-            q = np.float32(r[a] / np.float32(res))
-            key.append(int(math.floor(np.float32(q + origin[a]))))
-            fr.append(np.float32(q - np.float32(math.floor(q))))
-        k = tuple(key)
-        if k not in bins:
-            bins[k] = [np.float32(0)] * 3 + [0] + [np.float16(0)] * 4
-        b = bins[k]
-        for a in range(3):
-            b[a] = np.float32(b[a] + fr[a])
-            b[4 + a] = np.float16(b[4 + a] + np.float16(fr[a]))
-        b[3] += 1
-        b[7] = np.float16(b[7] + np.float16(1))
-    out = []
-    for k in sorted(bins):
-        b = bins[k]
-        if precision == 0:
-            c = np.float32(b[3])
-            pt = [np.float32(np.float32(b[a] / c) + np.float32(k[a])) for a in range(3)]
-        elif precision == 1:
-            c = np.float32(b[7])
-            pt = [np.float32(np.float32(np.float32(b[4 + a]) / c) + np.float32(k[a]))
-                  for a in range(3)]
-        else:
-            if precision == 3 and b[3] % 256 == 0:
+            assert abs(e[a] - float(mean[a])) < 2e-5, (k, a, e, mean)
+            assert k[a] - 1e-5 <= e[a] <= k[a] + 1 + 1e-5
+
+
+def test_voxel_filter_one_point_per_bin_is_that_point(oracle_lib):
+    # One point per bin: the endpoint is frac(q) + floor(q + o) with the
+    # eye's fractional part o (the bin carries the offset, the fraction
+    # does not, filterCloud.cu:119-128): q itself, or q + 1 on an axis
+    # where frac(q) + o >= 1.
+    rng = np.random.default_rng(52)
+    res = 0.5
+    cloud = _clustered(rng, 60, lambda k: 1, res, span=10)
+    f = _F([3.25, 7.5, -1.75], res=res, max_time=40.0)
+    o = np.array([e - math.floor(e) for e in f.eye])
+    rays = oracle.voxel_filter(f, cloud)
+    q = cloud.astype(np.float64) / res
+    expect = {}
+    for qi in q:
+        key = tuple(int(math.floor(qi[a] + o[a])) for a in range(3))
+        expect[key] = np.array([qi[a] - math.floor(qi[a]) + key[a]
+                                for a in range(3)])
+    assert len(rays) == len(expect) == 60
+    for k, e in zip(sorted(expect), _rays_endpoints(rays)):
+        assert np.allclose(e, expect[k], rtol=0, atol=2e-5), (k, e, expect[k])
+    # the offset carries (endpoint = q + 1) on some axis of some bins here
+    carried = sum(int(np.any(np.floor(qi + o) > np.floor(qi))) for qi in q)
+    assert 0 < carried < 60
+
+
+def test_voxel_filter_ray_count_is_distinct_bins(oracle_lib):
+    # Brute force on tiny clouds: the number of rays equals the number of
+    # distinct bins floor(q + o) (exact here: q and o have few bits), for
+    # every precision mode that emits a ray per bin.
+    rng = np.random.default_rng(53)
+    for trial in range(20):
+        res = 0.25
+        cloud = (rng.integers(-300, 300, size=(int(rng.integers(1, 40)), 3))
+                 / 64.0).astype(np.float32)
+        f = _F([5.25 + trial, -2.5, 0.75], res=res, max_time=60.0)
+        o = [e - math.floor(e) for e in f.eye]
+        keys = set()
+        for p in cloud.astype(np.float64):
+            if not 0 < np.linalg.norm(p) < 60.0 * res:
                 continue
-            pt = [np.float32(np.float32(k[a]) + np.float32(0.5)) for a in range(3)]
-        ln = np.sqrt(np.float32(np.float32(pt[0] * pt[0] + pt[1] * pt[1]) + pt[2] * pt[2]))
-        if ln > 0 and ln < np.float32(max_time):
-            out.append([pt[0] / ln, pt[1] / ln, pt[2] / ln, ln])
-    return np.array(out, np.float32).reshape(-1, 4)
+            keys.add(tuple(int(math.floor(p[a] / res + o[a]))
+                           for a in range(3)))
+        for prec in (0, 1, 2):
+            got = oracle.voxel_filter(f, cloud, precision=prec)
+            assert len(got) == len(keys), (trial, prec)
 
 
-def test_voxel_filter_matches_numpy_restatement(oracle_lib):
-    rng = np.random.default_rng(5)
-    cloud = rng.normal(size=(600, 3)).astype(np.float32) * np.float32(0.6)
-    cloud[::37] = np.nan
-    th = 0.3
-    R = [math.cos(th), -math.sin(th), 0, math.sin(th), math.cos(th), 0, 0, 0, 1]
-    f = _F([12.3, -4.7, 2.2], res=0.1, max_time=20.0)
-    f.rotation = [float(np.float32(x)) for x in R]
-    got = oracle.voxel_filter(f, cloud)
-    exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time)
-    assert got.shape == exp.shape and got.shape[0] > 50
-    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+def _box_for(width, height, vfov, rmin, rmax):
+    """FrustumData::calculateSensorBounds in closed form (float64)."""
+    hh = rmax * math.tan(math.radians(vfov) / 2)
+    hw = width / height * hh
+    return np.array([-hw, -hh, 0.0]), np.array([hw, hh, rmax])
 
 
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
-def test_voxel_filter_precision_modes(oracle_lib, precision):
-    # Scattered points plus dense voxels: 256 points (the 8-bit count
-    # wraps to 0), 300, and 2100 (the FP16 count stops at 2048).
-    rng = np.random.default_rng(17 + precision)
-    f = _F([12.3, -4.7, 2.2], res=0.1, max_time=20.0)
-    origin = np.array([e - math.floor(e) for e in f.eye])
-    parts = [rng.normal(size=(400, 3)).astype(np.float32) * np.float32(0.6)]
-    for n, k in ((256, (14, 8, 5)), (300, (-3, 4, 3)), (2100, (2, -4, -1))):
-        c = (np.asarray(k) + 0.5 - origin) * 0.1  # that voxel's centre
-        parts.append((c + rng.uniform(-0.03, 0.03, size=(n, 3))).astype(np.float32))
-    cloud = np.concatenate(parts)
-    got = oracle.voxel_filter(f, cloud, precision=precision)
-    exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time,
-                        precision)
-    assert got.shape == exp.shape and got.shape[0] > 50
-    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
-    if precision == 1:  # FP16 sums differ from FP32 ones in some bins
-        ref = oracle.voxel_filter(f, cloud, precision=0)
-        assert ref.shape == got.shape and not np.array_equal(ref, got)
-    if precision == 3:  # exactly the 256-point voxel is dropped
-        assert got.shape[0] == oracle.voxel_filter(f, cloud,
-                                                   precision=2).shape[0] - 1
+def test_voxel_filter_box_faces(oracle_lib, precision):
+    # The 640x480, 58 deg vertical FOV, 10 m sensor: box x within +-7.39 m
+    # (the aspect-derived half width), y within +-5.54 m, z in [0, 10].  A
+    # point exactly on a face is kept (the reference drops x < min and
+    # x > max), one ulp outside is dropped, one ulp inside kept.  Each
+    # probe point sits alone in its voxel, so the ray count says which.
+    lo, hi = _box_for(640, 480, 58.0, 0.0, 10.0)
+    lo = lo.astype(np.float32)
+    hi = hi.astype(np.float32)
+    res = 0.05
+    f = _F([0.5, 0.5, 0.5], res=res, max_time=400.0)  # range cap 20 m
+    f.box_min, f.box_max = list(lo), list(hi)
+    centre = np.float32([0.0, 0.0, 5.0])
+    kept, dropped = [], []
+    for a in range(3):
+        for face, out_dir in ((lo[a], -np.inf), (hi[a], np.inf)):
+            if a == 2 and face == 0.0:
+                continue  # z = 0 with x = y = 0 is the origin: |p| = 0
+            base = centre.copy()
+            if a == 2:
+                base[0] = np.float32(1.0)  # z-faces probed off the axis
+            p_on, p_out, p_in = base.copy(), base.copy(), base.copy()
+            p_on[a] = face
+            p_out[a] = np.nextafter(np.float32(face), np.float32(out_dir))
+            p_in[a] = np.nextafter(np.float32(face), np.float32(-out_dir))
+            kept += [p_on, p_in]
+            dropped += [p_out]
+    kept = np.array(kept, np.float32)
+    dropped = np.array(dropped, np.float32)
+    n_kept = len(oracle.voxel_filter(f, kept, precision=precision))
+    # face / inside pairs share a voxel: one ray per pair
+    assert n_kept == len(kept) // 2
+    assert len(oracle.voxel_filter(f, dropped, precision=precision)) == 0
+    both = np.concatenate([kept, dropped])
+    assert len(oracle.voxel_filter(f, both, precision=precision)) == n_kept
+    # without the box the outside probes count too (one voxel each)
+    g = _F(f.eye, res=res, max_time=400.0)
+    assert len(oracle.voxel_filter(g, dropped, precision=precision)) == \
+        len(dropped) == 5
+
+
+def test_voxel_filter_range_cap_is_strict(oracle_lib):
+    # 0 < |p| < max_range (metres): a point at exactly max_range is
+    # dropped, the float below it kept; the origin is dropped.
+    f = _F([0.5, 0.5, 0.5], res=0.25, max_time=50.0)  # 12.5 m of rays
+    f.max_range = 10.0
+    on = np.array([[0.0, 0.0, 10.0]], np.float32)
+    below = np.array([[0.0, 0.0, np.nextafter(np.float32(10.0),
+                                               np.float32(0))]], np.float32)
+    zero = np.zeros((1, 3), np.float32)
+    assert len(oracle.voxel_filter(f, on)) == 0
+    assert len(oracle.voxel_filter(f, below)) == 1
+    assert len(oracle.voxel_filter(f, zero)) == 0
+
+
+def test_voxel_filter_centre_modes(oracle_lib):
+    # PrecisionType 2 and 3 emit the voxel centre bin + 0.5; the 8-bit
+    # count of type 3 wraps, so exactly the bins with 256 k points vanish.
+    rng = np.random.default_rng(54)
+    res = 0.25
+    sizes = {1: 256, 2: 512, 3: 255, 4: 257, 5: 1, 6: 300}
+    cloud = _clustered(rng, 6, lambda k: sizes[k], res)
+    f = _F([4.0, 4.0, 4.0], res=res, max_time=60.0)
+    r2 = oracle.voxel_filter(f, cloud, precision=2)
+    r3 = oracle.voxel_filter(f, cloud, precision=3)
+    assert len(r2) == 6 and len(r3) == 4
+    for e in _rays_endpoints(r2):
+        assert np.allclose(e - np.floor(e), 0.5, atol=2e-5)
 
 
 def test_half_rounding_matches_numpy_float16(oracle_lib):
-    # Subnormal, tie and saturating-count cases of the FP16 sums, checked
-    # through single-point and repeated-point voxels.
-    f = _F([0.5, 0.5, 0.5], res=1.0, max_time=20.0)
+    # PrecisionType 1: each fraction rounded to FP16 and every sum, the
+    # count included, rounded to FP16 (half2 atomics in pixel order).
+    # numpy's float16 is an independent correctly rounded FP16: three equal
+    # points in one voxel give ((h + h) + h) / 3h + bin, with h = fp16(f);
+    # ties, a subnormal-free fraction and one that rounds up are covered.
+    f = _F([0.0, 0.0, 0.0], res=1.0, max_time=20.0)
     for x in (1.0 + 2.0 ** -12, 1.0 + 3 * 2.0 ** -12, 1.0 + 2.0 ** -20,
-              1.5 + 2.0 ** -11, 2.0 + 1e-7):
-        cloud = np.array([[x, 0.0, 0.0]] * 3, np.float32)
+              1.5 + 2.0 ** -11, 2.0 + 1e-7, 3.0 + 0.3333):
+        cloud = np.array([[x, 0.5, 0.5]] * 3, np.float32)
         got = oracle.voxel_filter(f, cloud, precision=1)
-        exp = _filter_numpy(cloud, f.rotation, f.eye, f.grid_res, f.max_time, 1)
-        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), x
+        fr = np.float32(np.float32(x) - np.float32(math.floor(x)))
+        h = np.float16(fr)
+        s = np.float16(np.float16(h + h) + h)
+        c = np.float16(np.float16(np.float16(1) + np.float16(1)) + np.float16(1))
+        ex = np.float32(np.float32(np.float32(s) / np.float32(c)) +
+                        np.float32(math.floor(x)))
+        e = _rays_endpoints(got)[0]
+        assert abs(e[0] - float(ex)) < 1e-6, (x, e[0], ex)
+    # the FP16 count stops at 2048: 2100 points count as 2048
+    cloud = np.array([[0.25, 0.5, 0.5]] * 2100, np.float32)
+    got = oracle.voxel_filter(f, cloud, precision=1)
+    h = np.float16(0)
+    for _ in range(2100):
+        h = np.float16(h + np.float16(1))
+    assert float(h) == 2048.0
+    e = _rays_endpoints(got)[0]
+    sx = np.float16(0)
+    for _ in range(2100):
+        sx = np.float16(sx + np.float16(0.25))
+    assert abs(e[0] - float(sx) / 2048.0) < 1e-6
 
 
 def test_frame_cast_totals_match_ray_traces(oracle_lib):
