@@ -7,7 +7,8 @@
  *      include/nanomap/handler/SimHandler.h:48,
  * whose grid is the agent map converted to NanoVDB (SimHandler.cpp:22-23)
  * and whose per-pixel ray is gpu::Sensor::getRay (include/nanomap/gpu/
- * Sensor.h:213-227).
+ * Sensor.h:119-156: the frustum branch 121-133 and the spherical laser
+ * branch 134-155, which the reference only runs on its CPU path).
  */
 #ifndef NANOMAP_AMD_SIM_H
 #define NANOMAP_AMD_SIM_H
@@ -22,10 +23,19 @@ extern "C" {
 
 typedef struct nm_simgrid nm_simgrid; /* device occupancy grid (5-4-3 tree) */
 
-/* One simulated frustum sensor.  rotation = normalized(q) * F (row-major);
- * hfov/vfov are independent (the reference derives hfov from the pixel
- * aspect, include/nanomap/sensor/FrustumData.h:24 -- pass
- * 2*atan(aspect*tan(vfov/2)) to reproduce it). */
+#define NM_SENSOR_FRUSTUM 0 /* pinhole depth camera (SharedParameters _type 0) */
+#define NM_SENSOR_LASER 1   /* spherical scanner (_type 1)                   */
+
+/* One simulated sensor.  rotation = normalized(q) * F (row-major).
+ * Frustum: width x height pixels; hfov/vfov are independent (the reference
+ * derives hfov from the pixel aspect, include/nanomap/sensor/
+ * FrustumData.h:24 -- pass 2*atan(aspect*tan(vfov/2)) to reproduce it);
+ * the range limits are z-depths (Sensor.h:129-130).
+ * Laser: width x height beams over hfov x vfov degrees (LaserData.h:23-26:
+ * width = (int)(hfov / angular step + 1)); beam (w, h) has yaw
+ * u = -hfov/2 + w hfov/width and pitch v = -vfov/2 + h vfov/height and
+ * direction (sin u cos v, sin v, cos u cos v) in the sensor frame
+ * (Sensor.h:134-155); the range limits are distances. */
 typedef struct nm_sim_sensor {
   float position[3];
   float rotation[9];
@@ -33,6 +43,7 @@ typedef struct nm_sim_sensor {
   int width, height;
   float hfov_deg, vfov_deg;
   float min_range, max_range;
+  int type; /* NM_SENSOR_* */
 } nm_sim_sensor;
 
 /* Occupancy grid of the voxels whose value exceeds `threshold`, from host
@@ -52,10 +63,13 @@ int nm_simgrid_info(nm_simgrid* grid, int64_t* active, int32_t bbox[6],
  * ijk (3 int32: first active voxel, INT32_MIN if no return), dist (metres
  * along the ray, -1 if none), xyz (sensor-frame point default_dir * dist,
  * NaN if none).  Any output may be NULL.  *returns receives the number of
- * returns.  Synchronizes `stream`. */
+ * returns.  Synchronizes `stream`.  The sensors' descriptors live in a
+ * device buffer owned by the grid, allocated when a call needs more room
+ * than any before it (never per call). */
 int nm_sim_cast(nm_simgrid* grid, const nm_sim_sensor* sensors,
                 int n_sensors, int32_t* ijk, float* dist, float* xyz,
                 int* returns, void* stream);
+
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,8 @@ import ctypes
 
 import numpy as np
 
-from ._capi import (FILTER_NONE, FILTER_VOXEL, FILTER_VOXEL_COUNT,
+from ._capi import (SENSOR_FRUSTUM, SENSOR_LASER,
+                    FILTER_NONE, FILTER_VOXEL, FILTER_VOXEL_COUNT,
                     FILTER_VOXEL_COUNT8, FILTER_VOXEL_HALF, ROUND_DOWN,
                     ROUND_NEAREST,
                     AgentFrame, BatchConfig, BatchTuning, Camera, CtxConfig,
@@ -436,8 +437,12 @@ def camera(width, height, hfov_deg, vfov_deg, max_range, noise_k=0.002,
 
 
 def sim_sensor(pos, rotation, grid_res, width, height, hfov_deg, vfov_deg,
-               min_range, max_range):
+               min_range, max_range, laser=False):
+    """A simulated sensor: a frustum camera of width x height pixels, or
+    with laser=True a spherical scanner of width x height beams over
+    hfov x vfov degrees (include/nanomap_amd_sim.h)."""
     s = SimSensor()
+    s.type = SENSOR_LASER if laser else SENSOR_FRUSTUM
     for k in range(3):
         s.position[k] = float(pos[k])
     for k, x in enumerate(np.asarray(rotation, np.float32).reshape(9)):

@@ -105,7 +105,12 @@ class SimSensor(ctypes.Structure):
                 ("rotation", ctypes.c_float * 9), ("grid_res", ctypes.c_float),
                 ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
-                ("min_range", ctypes.c_float), ("max_range", ctypes.c_float)]
+                ("min_range", ctypes.c_float), ("max_range", ctypes.c_float),
+                ("type", ctypes.c_int)]
+
+
+SENSOR_FRUSTUM = 0
+SENSOR_LASER = 1
 
 
 class Camera(ctypes.Structure):

@@ -1,10 +1,13 @@
 // nm_sim.hip -- sensor-simulation cast (generateCloud).
 //
 // Replaces viewSimKernel + getActiveCrossing (src/nanomap/kernels/
-// generateCloud.cu:20-82) and the frustum ray of gpu::Sensor::getRay
-// (include/nanomap/gpu/Sensor.h:213-227, set up by updateFrustum 156-171),
-// with an independent horizontal FOV (the reference derives it from the
-// pixel aspect, FrustumData.h:24).  FP32, IEEE round to nearest.
+// generateCloud.cu:20-82) and the rays of gpu::Sensor::getRay
+// (include/nanomap/gpu/Sensor.h:119-156, set up by updateFrustum 62-77):
+// the frustum branch with an independent horizontal FOV (the reference
+// derives it from the pixel aspect, FrustumData.h:24), and the spherical
+// laser branch.  FP32, IEEE round to nearest.  A laser beam's sines and
+// cosines come from per-sensor tables computed on the host, so every
+// arithmetic step on the device is a correctly rounded IEEE operation.
 //
 // The occupancy grid mirrors the NanoVDB tree the reference converts its
 // map into (SimHandler.cpp:22): 8^3 leaves with 512-bit active masks,
@@ -23,6 +26,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstring>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -39,6 +43,10 @@ struct nm_simgrid {
   int32_t* upper_child = nullptr;   // [n_upper*32768] lower index or -1
   int32_t* lower_child = nullptr;   // [n_lower*4096]  leaf index or -1
   uint32_t* leaf_mask = nullptr;    // [n_leaf*16]
+  // per-call inputs of nm_sim_cast (return counter, sensor descriptors,
+  // laser tables), grown when a call needs more than any before it
+  unsigned char* scratch = nullptr;
+  size_t scratch_bytes = 0;
 };
 
 namespace nm {
@@ -56,12 +64,15 @@ struct SimGridView {
 
 // Per-sensor constants, computed on the host exactly as the oracle does.
 struct SimDesc {
-  float T[3][3];  // transformed view vectors, T[column][component]
-  float I[3][3];  // identity view vectors
-  float s0, s1;   // 1/W, 1/H
+  float T[3][3];  // frustum: transformed view vectors, T[column][component]
+  float I[3][3];  // frustum: identity view vectors
+  float R[9];     // laser: rotation, row-major
+  float s0, s1;   // frustum: 1/W, 1/H
   float eye[3];   // position / res
   float res, min_range, max_range;
   int width, height;
+  int type;        // NM_SENSOR_*
+  int tab_off;     // laser: (sin, cos) of the W yaws, then of the H pitches
   int tile_off;    // first global tile of this sensor
   int tiles_x;     // tiles per row
   int64_t pix_off; // first output pixel of this sensor
@@ -132,14 +143,22 @@ __device__ __forceinline__ int fmin_index(float a, float b, float c) {
   return 2;
 }
 
+__device__ __forceinline__ float norm3(const float v[3]) {
+  return __builtin_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32(int* __restrict__ p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 __global__ __launch_bounds__(256) void k_sim_cast(
-    SimGridView g, const SimDesc* __restrict__ descs, int n_desc,
-    int total_tiles, int32_t* __restrict__ out_ijk,
-    float* __restrict__ out_dist, float* __restrict__ out_xyz,
-    int* __restrict__ returns) {
+    SimGridView g, const SimDesc* __restrict__ descs,
+    const float* __restrict__ tables, int n_desc, int total_tiles,
+    int32_t* __restrict__ out_ijk, float* __restrict__ out_dist,
+    float* __restrict__ out_xyz, int* __restrict__ returns) {
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (tile >= total_tiles) return;
+  if (tile >= total_tiles) return;  // whole wavefronts
   // sensor of this tile (few sensors: linear scan)
   int s = 0;
   while (s + 1 < n_desc && descs[s + 1].tile_off <= tile) ++s;
@@ -147,25 +166,53 @@ __global__ __launch_bounds__(256) void k_sim_cast(
   const int t = tile - d.tile_off;
   const int w = (t % d.tiles_x) * 8 + (lane & 7);
   const int h = (t / d.tiles_x) * 8 + (lane >> 3);
-  if (w >= d.width || h >= d.height) return;
+  const bool in_image = w < d.width && h < d.height;
   const int64_t pix = d.pix_off + (int64_t)h * d.width + w;
 
-  // gpu::Sensor::getRay, frustum branch
-  const float u = (float)w * d.s0, v = (float)h * d.s1;
-  float md[3], dd[3];
+  // gpu::Sensor::getRay.  md: the ray in the world frame, dd: the sensor
+  // frame direction the simulated point lies along.
+  float md[3], dd[3], min_t, max_t;
+  if (d.type == NM_SENSOR_LASER) {
+    // spherical branch (Sensor.h:134-155): yaw u, pitch v of the beam
+    // (lanes past the image edge read beam 0: they only join the ballot)
+    const int wc = in_image ? w : 0, hc = in_image ? h : 0;
+    const float su = tables[d.tab_off + 2 * wc];
+    const float cu = tables[d.tab_off + 2 * wc + 1];
+    const int ph = d.tab_off + 2 * d.width + 2 * hc;
+    const float sv = tables[ph], cv = tables[ph + 1];
+    dd[0] = su * cv;
+    dd[1] = sv;
+    dd[2] = cu * cv;
+    const float n0 = norm3(dd);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    md[i] = u * d.T[0][i] + v * d.T[1][i] - d.T[2][i];
-    dd[i] = u * d.I[0][i] + v * d.I[1][i] - d.I[2][i];
+    for (int i = 0; i < 3; ++i) dd[i] = dd[i] / n0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      md[i] = d.R[3 * i] * dd[0] + d.R[3 * i + 1] * dd[1] + d.R[3 * i + 2] * dd[2];
+    const float nm_ = norm3(md), nd = norm3(dd);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) md[i] = md[i] / nm_;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) dd[i] = dd[i] / nd;
+    min_t = d.min_range / d.res;
+    max_t = d.max_range / d.res;
+  } else {
+    // frustum branch (Sensor.h:121-133)
+    const float u = (float)w * d.s0, v = (float)h * d.s1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      md[i] = u * d.T[0][i] + v * d.T[1][i] - d.T[2][i];
+      dd[i] = u * d.I[0][i] + v * d.I[1][i] - d.I[2][i];
+    }
+    const float norm = norm3(md);
+    const float dn = norm3(dd);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) dd[i] = dd[i] / dn;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) md[i] = md[i] / norm;
+    min_t = (d.min_range / d.res) * norm;
+    max_t = (d.max_range / d.res) * norm;
   }
-  const float norm = __builtin_sqrtf(md[0] * md[0] + md[1] * md[1] + md[2] * md[2]);
-  const float dn = __builtin_sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) dd[i] = dd[i] / dn;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) md[i] = md[i] / norm;
-  const float min_t = (d.min_range / d.res) * norm;
-  const float max_t = (d.max_range / d.res) * norm;
   float eye[3], inv[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -176,7 +223,7 @@ __global__ __launch_bounds__(256) void k_sim_cast(
   bool hit = false;
   int vox[3] = {0, 0, 0};
   float thit = 0.0f;
-  if (g.active > 0) {
+  if (in_image && g.active > 0) {
     // Ray::clip(root bbox) over [min, max + 1)
     float t0 = 1.0e-5f, t1 = FLT_MAX;
     bool ok = true;
@@ -272,7 +319,12 @@ __global__ __launch_bounds__(256) void k_sim_cast(
       }
     }
   }
-  if (hit && thit >= min_t && thit <= max_t) {
+  const bool ret = in_image && hit && thit >= min_t && thit <= max_t;
+  // one atomic per wavefront for the return count
+  const uint64_t rb = __ballot(ret);
+  if (lane == 0 && rb) atomicAdd(returns, (int)__popcll(rb));
+  if (!in_image) return;
+  if (ret) {
     const float dist = thit * d.res;
     if (out_ijk) {
       out_ijk[3 * pix + 0] = vox[0];
@@ -285,7 +337,6 @@ __global__ __launch_bounds__(256) void k_sim_cast(
       out_xyz[3 * pix + 1] = dd[1] * dist;
       out_xyz[3 * pix + 2] = dd[2] * dist;
     }
-    atomicAdd(returns, 1);
   } else {
     if (out_ijk) {
       out_ijk[3 * pix + 0] = INT_MIN;
@@ -307,7 +358,135 @@ uint64_t okey(int x, int y, int z) {
          (uint64_t)((z >> 3) & m);
 }
 
+int setup_fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
 }  // namespace
+
+// Device layout of one call's inputs: the return counter at 0, the image
+// (descriptors, then laser tables) from kSimImageOffset.
+constexpr size_t kSimImageOffset = 256;
+
+struct SimLaunch {
+  int n_desc = 0;
+  int tiles = 0;
+  size_t tables_off = 0;  // byte offset of the tables inside the image
+};
+
+// Validates the sensors and lays out their descriptors and laser tables.
+// With img == null only *bytes (the image size) is computed.
+int sim_setup(const nm_sim_sensor* sensors, int n, unsigned char* img,
+              size_t cap, size_t* bytes, SimLaunch* L) {
+  size_t n_tab = 0;  // floats of laser tables
+  for (int k = 0; k < n; ++k) {
+    const nm_sim_sensor& in = sensors[k];
+    if (in.width <= 0 || in.height <= 0 || !(in.grid_res > 0) ||
+        (in.type != NM_SENSOR_FRUSTUM && in.type != NM_SENSOR_LASER) ||
+        (int64_t)in.width * in.height > (1 << 30))
+      return setup_fail(NM_EINVAL, "nm_sim_cast: bad sensor");
+    if (in.type == NM_SENSOR_LASER) n_tab += 2 * ((size_t)in.width + in.height);
+  }
+  const size_t tab_off = ((size_t)n * sizeof(SimDesc) + 15) & ~(size_t)15;
+  *bytes = tab_off + n_tab * sizeof(float);
+  if (!img) return NM_OK;
+  if (*bytes > cap) return setup_fail(NM_EINVAL, "nm_sim_cast: image too large");
+  SimDesc* descs = reinterpret_cast<SimDesc*>(img);
+  float* tab = reinterpret_cast<float*>(img + tab_off);
+  int tiles = 0, toff = 0;
+  int64_t pix = 0;
+  for (int k = 0; k < n; ++k) {
+    const nm_sim_sensor& in = sensors[k];
+    SimDesc d;
+    memset(&d, 0, sizeof(d));
+    d.type = in.type;
+    const float* R = in.rotation;
+    for (int i = 0; i < 9; ++i) d.R[i] = R[i];
+    if (in.type == NM_SENSOR_FRUSTUM) {
+      // Sensor::updateFrustum (Sensor.h:62-77) with an independent hfov
+      const float hh = (float)tan(in.vfov_deg * 3.14159265358979323846 / 360);
+      const float hw = (float)tan(in.hfov_deg * 3.14159265358979323846 / 360);
+      d.s0 = (float)(1.0 / in.width);
+      d.s1 = (float)(1.0 / in.height);
+      for (int i = 0; i < 3; ++i) {
+        const float r0 = R[3 * i + 0], r1 = R[3 * i + 1], r2 = R[3 * i + 2];
+        d.T[2][i] = hw * r0 + hh * r1 - r2;
+        d.T[0][i] = r0 * (2 * hw);
+        d.T[1][i] = r1 * (2 * hh);
+      }
+      const float e0[3] = {1, 0, 0}, e1[3] = {0, 1, 0}, e2[3] = {0, 0, 1};
+      for (int i = 0; i < 3; ++i) {
+        d.I[2][i] = hw * e0[i] + hh * e1[i] - e2[i];
+        d.I[0][i] = e0[i] * 2 * hw;
+        d.I[1][i] = e1[i] * 2 * hh;
+      }
+    } else {
+      // Sensor::getRay, laser branch (Sensor.h:134-143; scale from
+      // updateShared, Sensor.h:57-58): FP32 angles in degrees, scaled to
+      // radians through double; sin and cos of the FP32 angle, correctly
+      // rounded through double
+      d.tab_off = toff;
+      const float scale[2] = {in.hfov_deg / (float)in.width,
+                              in.vfov_deg / (float)in.height};
+      const float half[2] = {-in.hfov_deg / 2, -in.vfov_deg / 2};
+      const int count[2] = {in.width, in.height};
+      for (int a = 0; a < 2; ++a)
+        for (int i = 0; i < count[a]; ++i) {
+          volatile float ang = half[a] + (float)i * scale[a];
+          ang = (float)((double)ang * (3.14159265358979323846 / 180));
+          tab[toff++] = (float)sin((double)ang);
+          tab[toff++] = (float)cos((double)ang);
+        }
+    }
+    for (int a = 0; a < 3; ++a) d.eye[a] = in.position[a] / in.grid_res;
+    d.res = in.grid_res;
+    d.min_range = in.min_range;
+    d.max_range = in.max_range;
+    d.width = in.width;
+    d.height = in.height;
+    d.tiles_x = (in.width + 7) / 8;
+    d.tile_off = tiles;
+    d.pix_off = pix;
+    if ((int64_t)tiles + (int64_t)d.tiles_x * ((in.height + 7) / 8) > INT_MAX / 4)
+      return setup_fail(NM_EINVAL, "nm_sim_cast: too many pixels");
+    tiles += d.tiles_x * ((in.height + 7) / 8);
+    pix += (int64_t)in.width * in.height;
+    memcpy(&descs[k], &d, sizeof(d));
+  }
+  L->n_desc = n;
+  L->tiles = tiles;
+  L->tables_off = tab_off;
+  return NM_OK;
+}
+
+// Zeroes the return counter and casts every tile; scratch holds the
+// counter and, from kSimImageOffset, the image.
+hipError_t sim_launch(const nm_simgrid* g, const SimLaunch& L,
+                      unsigned char* scratch, int32_t* ijk, float* dist,
+                      float* xyz, hipStream_t s) {
+  SimGridView v;
+  for (int a = 0; a < 3; ++a) {
+    v.bmin[a] = g->bmin[a];
+    v.bmax[a] = g->bmax[a];
+  }
+  v.active = g->active;
+  v.n_upper = g->n_upper;
+  v.upper_origin = g->upper_origin;
+  v.upper_child = g->upper_child;
+  v.lower_child = g->lower_child;
+  v.leaf_mask = g->leaf_mask;
+  int* ret = reinterpret_cast<int*>(scratch);
+  const SimDesc* descs = reinterpret_cast<const SimDesc*>(scratch + kSimImageOffset);
+  const float* tables =
+      reinterpret_cast<const float*>(scratch + kSimImageOffset + L.tables_off);
+  k_zero_i32<<<1, 64, 0, s>>>(ret);
+  if (L.tiles > 0)
+    k_sim_cast<<<(L.tiles + 3) / 4, 256, 0, s>>>(v, descs, tables, L.n_desc,
+                                                 L.tiles, ijk, dist, xyz, ret);
+  return hipGetLastError();
+}
+
 }  // namespace nm
 
 namespace {
@@ -423,7 +602,9 @@ int nm_simgrid_from_map(nm_map* map, float threshold, nm_simgrid** out) {
 int nm_simgrid_destroy(nm_simgrid* g) {
   if (!g) return NM_OK;
   (void)hipSetDevice(g->device);
-  void* p[] = {g->upper_origin, g->upper_child, g->lower_child, g->leaf_mask};
+  (void)hipDeviceSynchronize();
+  void* p[] = {g->upper_origin, g->upper_child, g->lower_child, g->leaf_mask,
+               g->scratch};
   for (void* x : p)
     if (x) (void)hipFree(x);
   delete g;
@@ -454,73 +635,33 @@ int nm_sim_cast(nm_simgrid* g, const nm_sim_sensor* sensors, int n_sensors,
     return sim_fail(NM_EINVAL, "nm_sim_cast: bad argument");
   if (hipSetDevice(g->device) != hipSuccess)
     return sim_fail(NM_ENODEV, "nm_sim_cast: bad device");
-  std::vector<nm::SimDesc> descs(n_sensors);
-  int tiles = 0;
-  int64_t pix = 0;
-  for (int s = 0; s < n_sensors; ++s) {
-    const nm_sim_sensor& in = sensors[s];
-    if (in.width <= 0 || in.height <= 0 || !(in.grid_res > 0))
-      return sim_fail(NM_EINVAL, "nm_sim_cast: bad sensor");
-    nm::SimDesc& d = descs[s];
-    const float hh = (float)tan(in.vfov_deg * 3.14159265358979323846 / 360);
-    const float hw = (float)tan(in.hfov_deg * 3.14159265358979323846 / 360);
-    d.s0 = (float)(1.0 / in.width);
-    d.s1 = (float)(1.0 / in.height);
-    const float* R = in.rotation;
-    for (int i = 0; i < 3; ++i) {
-      const float r0 = R[3 * i + 0], r1 = R[3 * i + 1], r2 = R[3 * i + 2];
-      d.T[2][i] = hw * r0 + hh * r1 - r2;
-      d.T[0][i] = r0 * (2 * hw);
-      d.T[1][i] = r1 * (2 * hh);
-    }
-    const float e0[3] = {1, 0, 0}, e1[3] = {0, 1, 0}, e2[3] = {0, 0, 1};
-    for (int i = 0; i < 3; ++i) {
-      d.I[2][i] = hw * e0[i] + hh * e1[i] - e2[i];
-      d.I[0][i] = e0[i] * 2 * hw;
-      d.I[1][i] = e1[i] * 2 * hh;
-    }
-    for (int a = 0; a < 3; ++a) d.eye[a] = in.position[a] / in.grid_res;
-    d.res = in.grid_res;
-    d.min_range = in.min_range;
-    d.max_range = in.max_range;
-    d.width = in.width;
-    d.height = in.height;
-    d.tiles_x = (in.width + 7) / 8;
-    d.tile_off = tiles;
-    d.pix_off = pix;
-    tiles += d.tiles_x * ((in.height + 7) / 8);
-    pix += (int64_t)in.width * in.height;
-  }
+  size_t bytes = 0;
+  int st = nm::sim_setup(sensors, n_sensors, nullptr, 0, &bytes, nullptr);
+  if (st != NM_OK) return st;
+  std::vector<unsigned char> img(bytes);
+  nm::SimLaunch L;
+  st = nm::sim_setup(sensors, n_sensors, img.data(), bytes, &bytes, &L);
+  if (st != NM_OK) return st;
   hipStream_t s = (hipStream_t)stream;
-  nm::SimDesc* ddesc = nullptr;
-  int* dret = nullptr;
-  hipError_t e = hipMallocAsync((void**)&ddesc, sizeof(nm::SimDesc) * n_sensors, s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&dret, sizeof(int), s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(ddesc, descs.data(), sizeof(nm::SimDesc) * n_sensors,
-                       hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(dret, 0, sizeof(int), s);
-  if (e == hipSuccess) {
-    nm::SimGridView v;
-    for (int a = 0; a < 3; ++a) {
-      v.bmin[a] = g->bmin[a];
-      v.bmax[a] = g->bmax[a];
-    }
-    v.active = g->active;
-    v.n_upper = g->n_upper;
-    v.upper_origin = g->upper_origin;
-    v.upper_child = g->upper_child;
-    v.lower_child = g->lower_child;
-    v.leaf_mask = g->leaf_mask;
-    nm::k_sim_cast<<<(tiles + 3) / 4, 256, 0, s>>>(v, ddesc, n_sensors, tiles,
-                                                   ijk, dist, xyz, dret);
-    e = hipGetLastError();
+  const size_t need = nm::kSimImageOffset + bytes;
+  if (need > g->scratch_bytes) {
+    // a larger sensor set than any before: grow the grid's buffer
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        (g->scratch && hipFree(g->scratch) != hipSuccess))
+      return sim_fail(NM_EHIP, "nm_sim_cast: releasing the old buffer");
+    g->scratch = nullptr;
+    g->scratch_bytes = 0;
+    if (hipMalloc((void**)&g->scratch, need) != hipSuccess)
+      return sim_fail(NM_ENOMEM, "nm_sim_cast: descriptor buffer");
+    g->scratch_bytes = need;
   }
+  hipError_t e = hipMemcpyAsync(g->scratch + nm::kSimImageOffset, img.data(),
+                                bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(returns, dret, sizeof(int), hipMemcpyDeviceToHost, s);
+    e = nm::sim_launch(g, L, g->scratch, ijk, dist, xyz, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(returns, g->scratch, sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (ddesc) (void)hipFreeAsync(ddesc, s);
-  if (dret) (void)hipFreeAsync(dret, s);
   if (e != hipSuccess)
     return sim_fail(NM_EHIP, std::string("nm_sim_cast: ") + hipGetErrorString(e));
   return NM_OK;

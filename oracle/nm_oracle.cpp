@@ -555,9 +555,10 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
 // ---------------------------------------------------------------------
 // Sensor-simulation cast: restatement of viewSimKernel + getActiveCrossing
 // (generateCloud.cu:20-82) over a NanoVDB-style 5-4-3 tree, with the
-// frustum ray of gpu::Sensor::updateFrustum/getRay (Sensor.h:156-227)
+// rays of gpu::Sensor::updateFrustum/getRay (Sensor.h:62-77, 119-156)
 // taking an independent horizontal FOV (the reference derives it from the
-// pixel aspect, FrustumData.h:24, Sensor.h:160-161).  FP32, round to
+// pixel aspect, FrustumData.h:24, Sensor.h:64-67), and the spherical
+// laser branch (Sensor.h:134-155).  FP32, round to
 // nearest (the reference runs it on the GPU).
 // ---------------------------------------------------------------------
 }  // extern "C"
@@ -753,10 +754,27 @@ int64_t or_simgrid_active_count(const or_simgrid* g) { return g->active; }
 
 int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float res, int width, int height, float hfov_deg,
-                float vfov_deg, float min_range, float max_range,
+                float vfov_deg, float min_range, float max_range, int laser,
                 int hierarchical, int32_t* ijk, float* dist, float* xyz) {
   RoundGuard guard(0);
-  // Sensor::updateFrustum (Sensor.h:156-171)
+  // Laser beams (Sensor.h:134-143, scale from updateShared 57-58): yaw and
+  // pitch in FP32 degrees, -fov/2 + i fov/n, scaled to radians through
+  // double; sin and cos of the FP32 angle rounded from double.
+  std::vector<float> ls, lc;  // W yaws then H pitches
+  if (laser) {
+    const float fov[2] = {hfov_deg, vfov_deg};
+    const int n[2] = {width, height};
+    for (int a = 0; a < 2; ++a) {
+      const float step = fov[a] / (float)n[a];
+      for (int i = 0; i < n[a]; ++i) {
+        volatile float ang = -fov[a] / 2 + (float)i * step;
+        ang = (float)((double)ang * (3.14159265358979323846 / 180));
+        ls.push_back((float)std::sin((double)ang));
+        lc.push_back((float)std::cos((double)ang));
+      }
+    }
+  }
+  // Sensor::updateFrustum (Sensor.h:62-77)
   const float hh = (float)std::tan(vfov_deg * 3.14159265358979323846 / 360);
   const float hw = (float)std::tan(hfov_deg * 3.14159265358979323846 / 360);
   const float s0 = (float)(1.0 / width), s1 = (float)(1.0 / height);
@@ -777,19 +795,39 @@ int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
   for (int h = 0; h < height; ++h) {
     for (int w = 0; w < width; ++w) {
       const size_t pix = (size_t)h * width + w;
-      // Sensor::getRay, frustum branch (Sensor.h:215-227)
-      const float u = (float)w * s0, v = (float)h * s1;
-      float md[3], dd[3];
-      for (int i = 0; i < 3; ++i) {
-        md[i] = u * T[0][i] + v * T[1][i] - T[2][i];
-        dd[i] = u * I[0][i] + v * I[1][i] - I[2][i];
+      float md[3], dd[3], min_t, max_t;
+      auto len = [](const float* x) {
+        return std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      };
+      if (laser) {
+        // Sensor::getRay, laser branch (Sensor.h:134-155): the beam
+        // (sin u cos v, sin v, cos u cos v), normalised; the world ray is
+        // R times it, normalised; the point direction is normalised again
+        const float su = ls[w], cu = lc[w], sv = ls[width + h], cv = lc[width + h];
+        float b[3] = {su * cv, sv, cu * cv};
+        const float nb = len(b);
+        for (int i = 0; i < 3; ++i) b[i] = b[i] / nb;
+        for (int i = 0; i < 3; ++i)
+          md[i] = R[3 * i] * b[0] + R[3 * i + 1] * b[1] + R[3 * i + 2] * b[2];
+        const float nm = len(md), nd = len(b);
+        for (int i = 0; i < 3; ++i) md[i] = md[i] / nm;
+        for (int i = 0; i < 3; ++i) dd[i] = b[i] / nd;
+        min_t = min_range / res;
+        max_t = max_range / res;
+      } else {
+        // Sensor::getRay, frustum branch (Sensor.h:121-133)
+        const float u = (float)w * s0, v = (float)h * s1;
+        for (int i = 0; i < 3; ++i) {
+          md[i] = u * T[0][i] + v * T[1][i] - T[2][i];
+          dd[i] = u * I[0][i] + v * I[1][i] - I[2][i];
+        }
+        const float norm = len(md);
+        const float dn = len(dd);
+        for (int i = 0; i < 3; ++i) dd[i] = dd[i] / dn;
+        for (int i = 0; i < 3; ++i) md[i] = md[i] / norm;
+        min_t = (min_range / res) * norm;
+        max_t = (max_range / res) * norm;
       }
-      const float norm = std::sqrt(md[0] * md[0] + md[1] * md[1] + md[2] * md[2]);
-      const float dn = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-      for (int i = 0; i < 3; ++i) dd[i] = dd[i] / dn;
-      for (int i = 0; i < 3; ++i) md[i] = md[i] / norm;
-      const float min_t = (min_range / res) * norm;
-      const float max_t = (max_range / res) * norm;
       FRay r;
       for (int a = 0; a < 3; ++a) {
         r.eye[a] = pos[a] / res;

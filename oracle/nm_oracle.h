@@ -86,7 +86,7 @@ int or_voxel_filter(const unsigned char* cloud, int n_points, int step,
                     float max_time, float max_range, const float* box,
                     float* rays4, int max_rays, int precision);
 
-/* ---- sensor-simulation cast (generateCloud.cu:20-82, Sensor.h:156-227) - */
+/* ---- sensor-simulation cast (generateCloud.cu:20-82, Sensor.h:119-156) - */
 typedef struct or_simgrid or_simgrid;
 /* Occupancy grid: voxels with value > threshold of the given leaves
  * (origins: 3 ints/leaf, values: 512 floats/leaf).  Topology: a leaf exists
@@ -97,14 +97,15 @@ or_simgrid* or_simgrid_build(const int32_t* origins, const float* values,
                              int64_t n_leaves, float threshold);
 void or_simgrid_destroy(or_simgrid* g);
 int64_t or_simgrid_active_count(const or_simgrid* g);
-/* Cast a W x H frustum.  Per pixel (row-major): first active voxel ijk
- * (INT32_MIN if none in range), hit distance t*res in metres (-1 if none),
- * and the simulated sensor-frame point (NaN if no return).  hierarchical:
+/* Cast a W x H frustum (laser = 0) or a W x H-beam spherical scanner
+ * (laser = 1).  Per pixel (row-major): first active voxel ijk (INT32_MIN if
+ * none in range), hit distance t*res in metres (-1 if none), and the
+ * simulated sensor-frame point (NaN if no return).  hierarchical:
  * 1 = NanoVDB HDDA with node dims 1/8/128/4096, 0 = plain dim-1 DDA.
  * Returns the number of returns. */
 int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float res, int width, int height, float hfov_deg,
-                float vfov_deg, float min_range, float max_range,
+                float vfov_deg, float min_range, float max_range, int laser,
                 int hierarchical, int32_t* ijk, float* dist, float* xyz);
 
 /* ---- soft expected-depth march (include/nanomap_amd_soft.h), FP64 ------

@@ -109,3 +109,91 @@ def test_sim_empty_grid(gpu):
     s = nm.sim_sensor([0, 0, 0], np.eye(3), 0.1, 16, 16, 60.0, 45.0, 0.1, 10.0)
     ijk, dist, xyz, n = g.cast([s])
     assert n == 0 and bool((dist.cpu() == -1).all())
+
+
+def test_laser_and_frustum_sensors_bit_exact_and_mapped(gpu):
+    """Spherical scanners (the reference's laser getRay, Sensor.h:134-155)
+    beside a frustum camera in one call over the dense hall grid: ijk,
+    distance and point bit-exact against the oracle; the laser clouds then
+    map through the unfiltered cast (the reference maps laser clouds on its
+    CPU path, Handler.cpp:247-249) bit-exact against the oracle."""
+    nm = gpu
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=3)
+    om = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN)
+        oracle.process_frame(om, f, cloud)
+    o, v, _ = om.export()
+    gsim = nm.SimGrid(o, v, 0.0)
+    osim = oracle.SimGrid(o, v, 0.0)
+    kinds = [(901, 16, 360.0, 30.0, True), (320, 240, 87.0, 58.0, False),
+             (1024, 64, 360.0, 45.0, True), (361, 32, 180.0, 40.0, True)]
+    sens = []
+    for k, (w, h, hf, vf, laser) in enumerate(kinds):
+        pos, quat, _ = frames[k % 3]
+        fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 20.0, 0.85,
+                                -0.4, 0)
+        R = np.array(list(fr.rotation), np.float32)
+        sens.append((pos, quat, R, w, h, hf, vf, laser,
+                     nm.sim_sensor(pos, R, 0.05, w, h, hf, vf, 0.3, 20.0,
+                                   laser=laser)))
+    print("laser + frustum sim cast:", [k[:2] for k in kinds], flush=True)
+    ijk, dist, xyz, n = gsim.cast([s[-1] for s in sens])
+    ijk, dist, xyz = ijk.cpu().numpy(), dist.cpu().numpy(), xyz.cpu().numpy()
+    off = 0
+    total = 0
+    ctx = nm.Context(1024 * 64, 400.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for pos, quat, R, w, h, hf, vf, laser, _ in sens:
+        oi, od, ox, on = osim.cast(pos, R, 0.05, w, h, hf, vf, 0.3, 20.0,
+                                   laser=laser)
+        sl = slice(off, off + w * h)
+        assert np.array_equal(ijk[sl], oi), (w, h)
+        assert np.array_equal(dist[sl].view(np.uint32), od.view(np.uint32))
+        assert np.array_equal(xyz[sl].view(np.uint32), ox.view(np.uint32))
+        assert on > 1000
+        total += on
+        off += w * h
+        if laser:
+            f = W.frame(nm, pos, quat, 0.05, 20.0, nm.ROUND_DOWN)
+            ctx.submit(gmap, f, xyz[sl])
+            oracle.process_frame(omap, f, ox)
+    assert n == total
+    go, gv, gm = gmap.download()
+    oo, ov, omk = omap.export()
+    assert np.array_equal(go, oo) and len(go) > 100
+    assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
+    assert np.array_equal(gm, omk)
+
+
+def test_sim_64_clouds_fed_back(gpu):
+    """The 64 simulated 848x480 clouds of the box room (90x60 deg, 15 m)
+    mapped at 0.1 m through the plain path, bit-exact against the oracle."""
+    nm = gpu
+    gm_leaves = W.box_room_map_leaves(nm)
+    gsim = nm.SimGrid(gm_leaves[0], gm_leaves[1], 0.0)
+    poses = W.sim_poses(nm, 64)
+    sens = _sensors(nm, poses, 0.1, 848, 480, 90.0, 60.0, 0.1, 15.0)
+    print("sim feedback: 64 x 848x480", flush=True)
+    _, _, xyz, n = gsim.cast([s for _, _, s in sens])
+    xyz = xyz.cpu().numpy()
+    P = 848 * 480
+    ctx = nm.Context(P, 150.0)
+    gmap = nm.Map(0.1, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    rays = 0
+    for k, (pos, quat) in enumerate(poses):
+        f = W.frame(nm, pos, quat, 0.1, 15.0, nm.ROUND_DOWN, 848, 480)
+        c = xyz[k * P:(k + 1) * P]
+        ctx.submit(gmap, f, c)
+        rays += ctx.stats()["rays"]
+        oracle.process_frame(omap, f, c)
+    # (a return may lie past the mapper's range: the frustum's range limit
+    # is a z-depth, Sensor.h:129-130)
+    assert 1000 < rays <= n
+    go, gv, gm = gmap.download()
+    oo, ov, omk = omap.export()
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
+    assert np.array_equal(gm, omk)

@@ -130,3 +130,100 @@ def test_dense_grid_terminates(oracle_lib):
         R = np.array(list(fr.rotation), np.float32)
         _, d, _, n = g.cast(pos, R, 0.05, 320, 240, 87.0, 58.0, 0.1, 10.0)
         assert n > 40000
+
+
+def _wall_grid(axis, index, extent=64):
+    """Active voxels on the plane coordinate[axis] == index over
+    [-extent, extent) in the other two axes, plus one far voxel that keeps
+    the root bounding box wide."""
+    leaves = {}
+    other = [a for a in range(3) if a != axis]
+    for p in range(-extent, extent):
+        for q in range(-extent, extent):
+            v = [0, 0, 0]
+            v[axis] = index
+            v[other[0]], v[other[1]] = p, q
+            key = tuple(c & ~7 for c in v)
+            li = ((v[0] & 7) << 6) | ((v[1] & 7) << 3) | (v[2] & 7)
+            leaves.setdefault(key, np.zeros(512, np.float32))[li] = 1.0
+    far = (-200, -200, -200)
+    leaves.setdefault(far, np.zeros(512, np.float32))[0] = 1.0
+    o = np.array(list(leaves), np.int32)
+    v = np.stack([leaves[k] for k in leaves])
+    return oracle.SimGrid(o, v, 0.0)
+
+
+def test_laser_beams_closed_form(oracle_lib):
+    # 360 x 30 deg at 1 deg x 1.875 deg: beam w = 180 has yaw -180 + 180 = 0
+    # exactly, row h = 8 pitch -15 + 8 * 1.875 = 0: direction (0, 0, 1) in
+    # the sensor frame.  With R = I and a wall at z index 40 (res 0.1), a
+    # sensor at z = 0.25 m sees it at 4.0 - 0.25 m; the point is (0, 0, d).
+    # Beam w = 270 (yaw 90 deg) looks along +x: a wall at x index 30 from
+    # x = 0.55 m is at 3.0 - 0.55 m.  Pitch +-15 deg beams over a floor:
+    # d = h / sin(15 deg).
+    W, H = 360, 16
+    I = np.eye(3, dtype=np.float32)
+    g = _wall_grid(2, 40)
+    ijk, d, xyz, n = g.cast([0.05, 0.05, 0.25], I, 0.1, W, H, 360.0, 30.0,
+                            0.0, 20.0, laser=True)
+    pix = 8 * W + 180
+    assert tuple(ijk[pix]) == (0, 0, 40)
+    assert abs(d[pix] - 3.75) < 1e-5
+    assert np.allclose(xyz[pix], [0, 0, d[pix]], atol=1e-6)
+    # every return's point lies at its distance along a unit direction
+    ok = d >= 0
+    assert ok.sum() > 100
+    assert np.allclose(np.linalg.norm(xyz[ok], axis=1), d[ok], rtol=1e-6)
+    gx = _wall_grid(0, 30)
+    ijk, d, _, _ = gx.cast([0.55, 0.05, 0.05], I, 0.1, W, H, 360.0, 30.0,
+                           0.0, 20.0, laser=True)
+    pix = 8 * W + 270
+    assert tuple(ijk[pix]) == (30, 0, 0)
+    assert abs(d[pix] - 2.45) < 1e-5
+    # the floor below (y axis is "up" for the beam pitch): row 0 is pitch
+    # -15 deg; a wall at y index -20 from y = 0.05 m, beam yaw 0
+    # (voxel y = -20 spans [-2.0, -1.9) m: the beam enters its top face)
+    gy = _wall_grid(1, -20, extent=128)
+    ijk, d, _, _ = gy.cast([0.05, 0.05, 0.05], I, 0.1, W, H, 360.0, 30.0,
+                           0.0, 20.0, laser=True)
+    pix = 0 * W + 180
+    assert ijk[pix][1] == -20
+    expect = (0.05 + 1.9) / np.sin(np.radians(15.0))
+    assert abs(d[pix] - expect) < 1e-4
+    # ranges are distances: a max range below the wall gives no return
+    _, d2, _, _ = g.cast([0.05, 0.05, 0.25], I, 0.1, W, H, 360.0, 30.0, 0.0,
+                         3.5, laser=True)
+    assert d2[8 * W + 180] == -1
+
+
+def _hall_oracle_grid(n=3):
+    import nanomap_amd as nm
+    from nanomap_amd import workloads as W
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=n)
+    om = oracle.Map(*W.CLAMP)
+    for pos, quat, cloud in frames:
+        f = W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN)
+        oracle.process_frame(om, f, cloud)
+    o, v, _ = om.export()
+    return oracle.SimGrid(o, v, 0.0), frames
+
+
+def test_laser_hierarchical_matches_dim1(oracle_lib):
+    # a 901 x 16 beam scanner (0.4 x 2 deg) with the sample frame transform
+    # (beam y up = world z up) at the hall poses, over the 0.05 m hall map
+    import nanomap_amd as nm
+    g, frames = _hall_oracle_grid()
+    agree = total = 0
+    for pos, quat, _ in frames:
+        fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 30.0, 0.85,
+                                -0.4, 0)
+        R = np.array(list(fr.rotation), np.float32)
+        a_ijk, a_d, _, na = g.cast(pos, R, 0.05, 901, 16, 360.0, 30.0, 0.5,
+                                   30.0, True, laser=True)
+        b_ijk, b_d, _, nb = g.cast(pos, R, 0.05, 901, 16, 360.0, 30.0, 0.5,
+                                   30.0, False, laser=True)
+        same = np.all(a_ijk == b_ijk, axis=1)
+        agree += same.sum()
+        total += len(same)
+        assert na > 1000
+    assert agree / total > 0.999
