@@ -207,6 +207,38 @@ int nm_frame_submit(nm_ctx* ctx, nm_map* map, const nm_frame* frame,
                     const void* cloud, int n_points, int step,
                     int cloud_on_device, void* stream);
 
+/* ---- depth-image ingestion ---------------------------------------------
+ * A pinhole depth camera: pixel (u, v) of a width x height z-depth image
+ * (metres, row-major, NaN = no return) back-projects to the sensor-frame
+ * point (d ax[u], d ay[v], d), where ax[u] = ((u + 0.5) - cx) / fx and
+ * ay[v] = -((v + 0.5) - cy) / fy are each rounded once to FP32 and each
+ * coordinate is one FP32 product (optical frame: x right, y up, z
+ * forward, as the sample sensor's FrameTransform expects,
+ * config/sensors/640x480.txt:22-25).  The reference takes xyz clouds only
+ * (PointCloud::updateData, include/nanomap/gpu/PointCloud.h:121-134); a
+ * depth image is a third of the bytes to upload, and the conversion runs
+ * inside the cast's ray set-up. */
+typedef struct nm_depth_camera {
+  int width, height;
+  float fx, fy, cx, cy;
+} nm_depth_camera;
+/* fx = (W/2) / tan(hfov/2), fy = (H/2) / tan(vfov/2), cx = W/2, cy = H/2. */
+int nm_depth_camera_from_fov(int width, int height, float hfov_deg,
+                             float vfov_deg, nm_depth_camera* out);
+/* Host back-projection of a depth image into xyz (3 floats per pixel):
+ * the points the device computes, bit for bit. */
+int nm_depth_to_cloud(const nm_depth_camera* cam, const float* depth,
+                      float* xyz);
+/* nm_cast / nm_frame_submit of a depth image of cam->width x cam->height
+ * floats: a device pointer when depth_on_device, else host memory copied
+ * with one H2D inside the call. */
+int nm_cast_depth(nm_ctx* ctx, const nm_frame* frame,
+                  const nm_depth_camera* cam, const float* depth,
+                  int depth_on_device, void* stream);
+int nm_frame_submit_depth(nm_ctx* ctx, nm_map* map, const nm_frame* frame,
+                          const nm_depth_camera* cam, const float* depth,
+                          int depth_on_device, void* stream);
+
 /* Stats of the last frame.  Synchronizes the context's stream. */
 int nm_frame_stats_get(nm_ctx* ctx, nm_frame_stats* out);
 

@@ -15,12 +15,14 @@ from ._capi import (SENSOR_FRUSTUM, SENSOR_LASER,
                     FILTER_VOXEL_COUNT8, FILTER_VOXEL_HALF, ROUND_DOWN,
                     ROUND_NEAREST,
                     AgentFrame, BatchConfig, BatchTuning, Camera, CtxConfig,
+                    DepthCamera,
                     Frame, FrameStats, NanoMapError, SimSensor, _ptr, check,
                     lib)
 
 __all__ = ["Context", "Map", "Batch", "Scene", "Camera", "Frame", "FrameStats",
            "SimGrid", "SimSensor", "sim_sensor",
-           "frame_from_pose", "frustum_box", "fold", "NanoMapError", "ROUND_DOWN",
+           "frame_from_pose", "frustum_box", "depth_camera",
+           "depth_to_cloud", "fold", "NanoMapError", "ROUND_DOWN",
            "ROUND_NEAREST", "FILTER_NONE", "FILTER_VOXEL", "FILTER_VOXEL_HALF",
            "FILTER_VOXEL_COUNT", "FILTER_VOXEL_COUNT8", "DEFAULT_F"]
 
@@ -69,6 +71,27 @@ def frame_from_pose(pos, quat, F=DEFAULT_F, grid_res=0.1, max_range=5.0,
             f.box_min[a] = float(box[0][a])
             f.box_max[a] = float(box[1][a])
     return f
+
+
+def depth_camera(width, height, hfov_deg, vfov_deg):
+    """Pinhole depth camera with fx, fy from the FOVs and the principal
+    point at the image centre (nm_depth_camera_from_fov)."""
+    c = DepthCamera()
+    check(lib().nm_depth_camera_from_fov(int(width), int(height),
+                                         float(hfov_deg), float(vfov_deg),
+                                         ctypes.byref(c)))
+    return c
+
+
+def depth_to_cloud(cam, depth):
+    """Host back-projection of a depth image [H*W] into xyz [H*W, 3]: the
+    points the device computes from the same image, bit for bit."""
+    d = np.ascontiguousarray(depth, np.float32).reshape(-1)
+    out = np.zeros((d.size, 3), np.float32)
+    check(lib().nm_depth_to_cloud(ctypes.byref(cam),
+                                  d.ctypes.data_as(ctypes.c_void_p),
+                                  out.ctypes.data_as(ctypes.c_void_p)))
+    return out
 
 
 class Context:
@@ -122,6 +145,28 @@ class Context:
         self._keep = cloud
         check(lib().nm_frame_submit(self.handle, m.handle, ctypes.byref(frame),
                                     ptr, n, st, dev, stream))
+
+    @staticmethod
+    def _depth_args(depth):
+        if hasattr(depth, "data_ptr"):  # torch tensor
+            return ctypes.c_void_p(depth.data_ptr()), int(depth.is_cuda)
+        a = np.ascontiguousarray(depth, dtype=np.float32)
+        return a.ctypes.data_as(ctypes.c_void_p), 0
+
+    def cast_depth(self, frame, cam, depth, stream=None):
+        """nm_cast of a depth image (cam.width x cam.height floats)."""
+        ptr, dev = self._depth_args(depth)
+        self._keep = depth
+        check(lib().nm_cast_depth(self.handle, ctypes.byref(frame),
+                                  ctypes.byref(cam), ptr, dev, stream))
+
+    def submit_depth(self, m, frame, cam, depth, stream=None):
+        ptr, dev = self._depth_args(depth)
+        self._keep = depth
+        check(lib().nm_frame_submit_depth(self.handle, m.handle,
+                                          ctypes.byref(frame),
+                                          ctypes.byref(cam), ptr, dev,
+                                          stream))
 
     def stats(self):
         s = FrameStats()

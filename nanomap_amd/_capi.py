@@ -56,6 +56,12 @@ class Frame(ctypes.Structure):
         return f
 
 
+class DepthCamera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("fx", ctypes.c_float), ("fy", ctypes.c_float),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float)]
+
+
 class FrameStats(ctypes.Structure):
     _fields_ = [("points", ctypes.c_int64), ("rays", ctypes.c_int64),
                 ("visits", ctypes.c_int64), ("voxels", ctypes.c_int64),
@@ -157,6 +163,13 @@ SIGNATURES = {
     "nm_frame_submit": (_I, [_P, _P, ctypes.POINTER(Frame), _P, _I, _I, _I,
                              _P]),
     "nm_frame_stats_get": (_I, [_P, ctypes.POINTER(FrameStats)]),
+    "nm_depth_camera_from_fov": (_I, [_I, _I, _F, _F,
+                                      ctypes.POINTER(DepthCamera)]),
+    "nm_depth_to_cloud": (_I, [ctypes.POINTER(DepthCamera), _P, _P]),
+    "nm_cast_depth": (_I, [_P, ctypes.POINTER(Frame),
+                           ctypes.POINTER(DepthCamera), _P, _I, _P]),
+    "nm_frame_submit_depth": (_I, [_P, _P, ctypes.POINTER(Frame),
+                                   ctypes.POINTER(DepthCamera), _P, _I, _P]),
     "nm_counts_download": (_I, [_P, ctypes.c_int64, _PI64, _PI32, _PU32,
                                 _PU32]),
     "nm_filter_rays_download": (_I, [_P, _I, ctypes.POINTER(_I), _PF]),

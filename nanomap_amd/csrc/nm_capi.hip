@@ -56,6 +56,12 @@ struct nm_ctx {
   // the workspace's miss-run table: valid for (miss bits, rounding)
   bool runs_valid = false;
   uint64_t runs_key = 0;
+  // depth-image factors (ax[0, W), ay[0, H)) of the last depth camera
+  float* depth_tab = nullptr;
+  int depth_tab_cap = 0;
+  bool depth_valid = false;
+  nm_depth_camera depth_cam{};
+  std::vector<float> depth_host;
   // stage timing
   bool timing = false;
   unsigned timing_mask = ~0u;  // stages timed while timing is on
@@ -151,6 +157,7 @@ void free_ctx(nm_ctx* c) {
     (void)hipEventDestroy(m.b);
   }
   for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
+  if (c->depth_tab) (void)hipFree(c->depth_tab);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   if (c->join_ev) (void)hipEventDestroy(c->join_ev);
@@ -395,13 +402,98 @@ int nm_map_device_arrays(nm_map* m, void** values, void** masks,
   return NM_OK;
 }
 
+namespace {
+
+// ax[u] = ((u + 0.5) - cx) / fx, ay[v] = -((v + 0.5) - cy) / fy, each
+// evaluated in double and rounded once to FP32.
+void depth_factors(const nm_depth_camera& c, float* ax, float* ay) {
+  for (int u = 0; u < c.width; ++u)
+    ax[u] = (float)((((double)u + 0.5) - (double)c.cx) / (double)c.fx);
+  for (int v = 0; v < c.height; ++v)
+    ay[v] = (float)(-(((double)v + 0.5) - (double)c.cy) / (double)c.fy);
+}
+
+bool depth_camera_ok(const nm_depth_camera* c) {
+  return c && c->width > 0 && c->height > 0 &&
+         (int64_t)c->width * c->height <= (1 << 30) && std::isfinite(c->fx) &&
+         std::isfinite(c->fy) && c->fx != 0.0f && c->fy != 0.0f &&
+         std::isfinite(c->cx) && std::isfinite(c->cy);
+}
+
+int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
+              int n_points, int step, int cloud_on_device,
+              const nm_depth_camera* cam, void* stream);
+
+}  // namespace
+
+int nm_depth_camera_from_fov(int width, int height, float hfov_deg,
+                             float vfov_deg, nm_depth_camera* out) {
+  if (!out || width <= 0 || height <= 0 || !(hfov_deg > 0 && hfov_deg < 180) ||
+      !(vfov_deg > 0 && vfov_deg < 180))
+    return fail(NM_EINVAL, "nm_depth_camera_from_fov: bad argument");
+  out->width = width;
+  out->height = height;
+  out->fx = (float)(0.5 * width / tan(hfov_deg * M_PI / 360.0));
+  out->fy = (float)(0.5 * height / tan(vfov_deg * M_PI / 360.0));
+  out->cx = 0.5f * (float)width;
+  out->cy = 0.5f * (float)height;
+  return NM_OK;
+}
+
+int nm_depth_to_cloud(const nm_depth_camera* cam, const float* depth,
+                      float* xyz) {
+  if (!depth_camera_ok(cam) || !depth || !xyz)
+    return fail(NM_EINVAL, "nm_depth_to_cloud: bad argument");
+  std::vector<float> ax(cam->width), ay(cam->height);
+  depth_factors(*cam, ax.data(), ay.data());
+  for (int v = 0; v < cam->height; ++v)
+    for (int u = 0; u < cam->width; ++u) {
+      const size_t i = (size_t)v * cam->width + u;
+      const float d = depth[i];
+      xyz[3 * i + 0] = d * ax[u];
+      xyz[3 * i + 1] = d * ay[v];
+      xyz[3 * i + 2] = d;
+    }
+  return NM_OK;
+}
+
+int nm_cast_depth(nm_ctx* ctx, const nm_frame* frame,
+                  const nm_depth_camera* cam, const float* depth,
+                  int depth_on_device, void* stream) {
+  if (!ctx || !frame || !depth_camera_ok(cam))
+    return fail(NM_EINVAL, "nm_cast_depth: bad argument");
+  return cast_impl(ctx, frame, depth, cam->width * cam->height, 4,
+                   depth_on_device, cam, stream);
+}
+
+int nm_frame_submit_depth(nm_ctx* ctx, nm_map* map, const nm_frame* frame,
+                          const nm_depth_camera* cam, const float* depth,
+                          int depth_on_device, void* stream) {
+  int st = nm_cast_depth(ctx, frame, cam, depth, depth_on_device, stream);
+  if (st != NM_OK) return st;
+  return nm_update_map(ctx, map, stream);
+}
+
 int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
             int n_points, int step, int cloud_on_device, void* stream) {
+  return cast_impl(ctx, frame, cloud, n_points, step, cloud_on_device,
+                   nullptr, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
+              int n_points, int step, int cloud_on_device,
+              const nm_depth_camera* cam, void* stream) {
   if (!ctx || !frame) return fail(NM_EINVAL, "nm_cast: null argument");
   if (n_points < 0 || n_points > ctx->max_points)
     return fail(NM_EINVAL, "nm_cast: n_points exceeds the context capacity");
-  if (n_points > 0 && (!cloud || step < 12 || (step & 3)))
+  if (!cam && n_points > 0 && (!cloud || step < 12 || (step & 3)))
     return fail(NM_EINVAL, "nm_cast: cloud must be xyz f32 with a stride >= 12, multiple of 4");
+  if (cam && n_points > 0 && !cloud)
+    return fail(NM_EINVAL, "nm_cast_depth: null depth image");
   if (!(frame->grid_res > 0) || !(frame->max_time > 0) ||
       frame->max_time > ctx->max_time)
     return fail(NM_EINVAL, "nm_cast: max_time must be in (0, context max_time]");
@@ -441,6 +533,30 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
     pts = ctx->b.cloud;
   }
 
+  if (cam) {
+    // the camera's factor tables, uploaded when the camera changes
+    const bool same = ctx->depth_valid &&
+                      memcmp(&ctx->depth_cam, cam, sizeof(*cam)) == 0;
+    if (!same) {
+      const int need = cam->width + cam->height;
+      NM_HIP(hipStreamSynchronize(s));  // the previous upload has finished
+      if (need > ctx->depth_tab_cap) {
+        if (ctx->depth_tab) NM_HIP(hipFree(ctx->depth_tab));
+        ctx->depth_tab = nullptr;
+        ctx->depth_tab_cap = 0;
+        NM_HIP(hipMalloc((void**)&ctx->depth_tab, need * sizeof(float)));
+        ctx->depth_tab_cap = need;
+      }
+      ctx->depth_host.resize(need);
+      depth_factors(*cam, ctx->depth_host.data(),
+                    ctx->depth_host.data() + cam->width);
+      NM_HIP(hipMemcpyAsync(ctx->depth_tab, ctx->depth_host.data(),
+                            need * sizeof(float), hipMemcpyHostToDevice, s));
+      ctx->depth_cam = *cam;
+      ctx->depth_valid = true;
+    }
+  }
+
   nm::Job p{};
   memcpy(p.R, frame->rotation, sizeof(p.R));
   memcpy(p.eye, frame->eye, sizeof(p.eye));
@@ -452,10 +568,13 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.cloud = pts;
   p.n = n_points;
   p.step = step;
-  p.width = frame->width;
-  p.height = frame->height;
-  p.tiled = (frame->width > 0 && frame->height > 0 &&
-             (int64_t)frame->width * frame->height == n_points &&
+  p.width = cam ? cam->width : frame->width;
+  p.height = cam ? cam->height : frame->height;
+  p.depth = cam ? 1 : 0;
+  p.depth_ax = cam ? ctx->depth_tab : nullptr;
+  p.depth_ay = cam ? ctx->depth_tab + cam->width : nullptr;
+  p.tiled = (p.width > 0 && p.height > 0 &&
+             (int64_t)p.width * p.height == n_points &&
              frame->filter == NM_FILTER_NONE)
                 ? 1
                 : 0;
@@ -526,6 +645,10 @@ int nm_cast(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   ctx->stream = s;
   return NM_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int nm_update_map(nm_ctx* ctx, nm_map* map, void* stream) {
   if (!ctx || !map) return fail(NM_EINVAL, "nm_update_map: null argument");

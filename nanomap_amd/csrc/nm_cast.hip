@@ -83,9 +83,9 @@ __device__ __forceinline__ bool ray_of(const Job& p, int i, RayDir& r) {
     r.L = L;
     return true;
   }
-  const float* pt =
-      reinterpret_cast<const float*>(p.cloud + (size_t)i * (size_t)p.step);
-  return ray_from_point<M>(p.R, pt[0], pt[1], pt[2], p.res, p.max_time, r);
+  float x, y, z;
+  load_point(p, i, x, y, z);
+  return ray_from_point<M>(p.R, x, y, z, p.res, p.max_time, r);
 }
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {

@@ -38,9 +38,8 @@ __device__ __forceinline__ void point_bin(const CastParams& p,
                                           const FilterBounds& fb,
                                           const float* origin, int i,
                                           bool& valid, int b[3], float f[3]) {
-  const float* pt =
-      reinterpret_cast<const float*>(p.cloud + (size_t)i * (size_t)p.step);
-  const float px = pt[0], py = pt[1], pz = pt[2];
+  float px, py, pz;
+  load_point(p, i, px, py, pz);
   valid = false;
   if (__builtin_isnan(px) || __builtin_isnan(py) || __builtin_isnan(pz))
     return;

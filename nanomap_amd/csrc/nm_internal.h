@@ -124,7 +124,10 @@ struct Job {
   float hit;
   float miss;
   Cube cube;
-  const unsigned char* cloud;  // xyz f32 points with byte stride `step`
+  const unsigned char* cloud;  // xyz f32 points with byte stride `step`,
+                               // or (depth) a width x height f32 depth image
+  const float* depth_ax;       // depth image: x factor of each column
+  const float* depth_ay;       // depth image: y factor of each row
   const float* rays4;          // or pre-filtered rays (dir, length)
   const int32_t* n_dev;        // device count of rays4 entries
   int n;                       // points (upper bound on rays)
@@ -135,14 +138,36 @@ struct Job {
   int use_rays;                // 1: cast rays4 instead of points
   int layout;                  // Layout of the work items (tiled only)
   int map;                     // index into the map table
+  int depth;                   // 1: `cloud` is a depth image
   // placement in the shared workspace
   int64_t cube_off;  // first entry of the job's cube table (kScanChunk-aligned)
   int ray_off;       // global index of the job's first point
   int item_off;      // first cast work item of the job
   int nitems;        // cast work items of the job
-  int pad_;
 };
 using CastParams = Job;
+
+// Sensor-frame point i of a job: the i-th xyz point of the cloud, or from
+// a depth image the pinhole back-projection (d ax[u], d ay[v], d) of pixel
+// (u, v) = (i % width, i / width) with z-depth d (FP32 products; NaN
+// depth gives a NaN point).
+__device__ __forceinline__ void load_point(const Job& p, int i, float& x,
+                                           float& y, float& z) {
+  if (p.depth) {
+    const float d = reinterpret_cast<const float*>(p.cloud)[i];
+    const int v = i / p.width;
+    const int u = i - v * p.width;
+    x = d * p.depth_ax[u];
+    y = d * p.depth_ay[v];
+    z = d;
+    return;
+  }
+  const float* pt =
+      reinterpret_cast<const float*>(p.cloud + (size_t)i * (size_t)p.step);
+  x = pt[0];
+  y = pt[1];
+  z = pt[2];
+}
 
 // How the cast kernels walk their work items (chosen by the autotuner for
 // batches; plain frames use the default).

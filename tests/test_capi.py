@@ -225,3 +225,33 @@ def test_map_load_rejects_bad_files_before_any_device_work(nm, tmp_path):
         nm.Map.load(p)
     with pytest.raises(nm.NanoMapError):
         nm.Map.load(tmp_path / "missing.nmap")
+
+
+def test_depth_back_projection_closed_forms(nm):
+    # (d ax[u], d ay[v], d): z is the depth itself; columns u and W-1-u
+    # mirror exactly (cx = W/2); the corner ray's angle is the half FOV
+    # less half a pixel; NaN depth gives a NaN point
+    W_, H_ = 640, 480
+    cam = nm.depth_camera(W_, H_, 87.0, 58.0)
+    assert cam.cx == 320.0 and cam.cy == 240.0
+    d = np.full(W_ * H_, 2.5, np.float32)
+    d[7] = np.nan
+    xyz = nm.depth_to_cloud(cam, d).reshape(H_, W_, 3)
+    assert np.all(xyz[..., 2].reshape(-1)[np.arange(W_ * H_) != 7] == 2.5)
+    assert np.isnan(xyz[0, 7]).all()
+    assert np.array_equal(xyz[5, :, 0], -xyz[5, ::-1, 0])
+    assert np.array_equal(xyz[:, 9, 1], -xyz[::-1, 9, 1])
+    ang = np.degrees(np.arctan2(xyz[100, W_ - 1, 0], 2.5))
+    want = np.degrees(np.arctan((W_ / 2 - 0.5) / cam.fx))
+    assert abs(ang - want) < 1e-5 and abs(want - 43.5) < 0.1
+    assert xyz[0, 0, 1] > 0 > xyz[H_ - 1, 0, 1]  # row 0 is the top: y up
+    # the scene renderer's cloud is the same pinhole: its z is the depth
+    from nanomap_amd import workloads
+    _, cam2, frames = workloads.box_room_frame(nm)
+    cloud = frames[0][2]
+    c = nm.depth_camera(cam2.width, cam2.height, cam2.hfov_deg, cam2.vfov_deg)
+    back = nm.depth_to_cloud(c, cloud[:, 2].copy())
+    ok = ~np.isnan(cloud[:, 2])
+    assert np.allclose(back[ok], cloud[ok], rtol=0, atol=2e-6)
+    with pytest.raises(nm.NanoMapError):
+        nm.depth_camera(0, 480, 87.0, 58.0)

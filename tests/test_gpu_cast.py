@@ -458,3 +458,38 @@ def test_stage_timing_all_and_masked(gpu):
         assert np.array_equal(o, maps[0][0])
         assert np.array_equal(v.view(np.uint32), maps[0][1].view(np.uint32))
         assert np.array_equal(mk, maps[0][2])
+
+
+def test_depth_images_bit_exact(gpu):
+    # the hall frames as z-depth images (a third of the bytes of the xyz
+    # clouds), back-projected inside the cast's ray set-up: counts and map
+    # bit-exact against the oracle fed the host back-projection of the same
+    # images; host and device images, with and without the voxel filter
+    import torch
+    nm = gpu
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=4)
+    dc = nm.depth_camera(cam.width, cam.height, cam.hfov_deg, cam.vfov_deg)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP)
+    omap = oracle.Map(*W.CLAMP)
+    for k, (pos, quat, cloud) in enumerate(frames):
+        depth = np.ascontiguousarray(cloud[:, 2])
+        filt = nm.FILTER_VOXEL if k == 3 else nm.FILTER_NONE
+        f = W.frame(nm, pos, quat, 0.05, 10.0, RD, cam.width, cam.height,
+                    filt, box=W.hall_box(nm, RD) if filt else None)
+        print("depth image %d: filter %d" % (k, filt), flush=True)
+        src = torch.from_numpy(depth).cuda() if k % 2 else depth
+        ctx.cast_depth(f, dc, src)
+        xyz = nm.depth_to_cloud(dc, depth)
+        temp = oracle.Temp()
+        if filt:
+            rays = oracle.voxel_filter(f, xyz)
+            g_rays = ctx.filter_rays()
+            assert np.array_equal(g_rays.view(np.uint32), rays.view(np.uint32))
+            temp.cast_rays(f, rays)
+        else:
+            temp.cast_cloud(f, xyz)
+        _check_counts(ctx, temp)
+        ctx.update(gmap)
+        omap.integrate(temp, RD)
+    _check_maps(gmap, omap)
