@@ -70,6 +70,21 @@ int nm_sim_cast(nm_simgrid* grid, const nm_sim_sensor* sensors,
                 int n_sensors, int32_t* ijk, float* dist, float* xyz,
                 int* returns, void* stream);
 
+/* nm_sim_cast without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The sensor descriptors and the return counter go to a fixed device
+ * buffer of NM_SIM_STATIC_BYTES per GPU, held until the process ends;
+ * sensor sets that do not fit (about 300 frustum sensors, fewer lasers:
+ * each laser needs 8 bytes per beam column and row) are rejected with
+ * NM_EINVAL.  Calls on one GPU share that buffer: calls that overlap on
+ * one GPU (other threads, other streams) race.  Same results as
+ * nm_sim_cast. */
+#define NM_SIM_STATIC_BYTES 65536
+int nm_sim_cast_static(nm_simgrid* grid, const nm_sim_sensor* sensors,
+                       int n_sensors, int32_t* ijk, float* dist, float* xyz,
+                       int* returns, void* stream);
+
 
 #ifdef __cplusplus
 }

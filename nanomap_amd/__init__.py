@@ -537,10 +537,11 @@ class SimGrid:
                                     _ptr(cn, ctypes.c_int32)))
         return {"active": a.value, "bbox": bb.tolist(), "nodes": cn.tolist()}
 
-    def cast(self, sensors, stream=None):
+    def cast(self, sensors, stream=None, static=False):
         """Simulate the sensors; returns device torch tensors
         (ijk [N,3] int32, dist [N] f32, xyz [N,3] f32) and the return
-        count, N = total pixels."""
+        count, N = total pixels.  static=True takes the allocation-free
+        entry point (nm_sim_cast_static: fixed per-GPU descriptor buffer)."""
         import torch
         if isinstance(sensors, SimSensor):
             sensors = [sensors]
@@ -550,9 +551,9 @@ class SimGrid:
         dist = torch.empty(N, dtype=torch.float32, device="cuda")
         xyz = torch.empty((N, 3), dtype=torch.float32, device="cuda")
         n = ctypes.c_int()
-        check(lib().nm_sim_cast(self.handle, arr, len(sensors),
-                                ctypes.c_void_p(ijk.data_ptr()),
-                                ctypes.c_void_p(dist.data_ptr()),
-                                ctypes.c_void_p(xyz.data_ptr()),
-                                ctypes.byref(n), stream))
+        fn = lib().nm_sim_cast_static if static else lib().nm_sim_cast
+        check(fn(self.handle, arr, len(sensors),
+                 ctypes.c_void_p(ijk.data_ptr()),
+                 ctypes.c_void_p(dist.data_ptr()),
+                 ctypes.c_void_p(xyz.data_ptr()), ctypes.byref(n), stream))
         return ijk, dist, xyz, n.value

@@ -117,6 +117,7 @@ class SimSensor(ctypes.Structure):
 
 SENSOR_FRUSTUM = 0
 SENSOR_LASER = 1
+SIM_STATIC_BYTES = 65536  # NM_SIM_STATIC_BYTES
 
 
 class Camera(ctypes.Structure):
@@ -216,6 +217,8 @@ SIGNATURES = {
     "nm_simgrid_info": (_I, [_P, _PI64, _PI32, _PI32]),
     "nm_sim_cast": (_I, [_P, ctypes.POINTER(SimSensor), _I, _P, _P, _P,
                          ctypes.POINTER(_I), _P]),
+    "nm_sim_cast_static": (_I, [_P, ctypes.POINTER(SimSensor), _I, _P, _P,
+                                _P, ctypes.POINTER(_I), _P]),
 }
 
 _lib = None

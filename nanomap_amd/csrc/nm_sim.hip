@@ -27,6 +27,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -496,6 +497,39 @@ int sim_fail(int code, const std::string& msg) {
 }
 }  // namespace
 
+// The buffer of nm_sim_cast_static: the return counter and the sensors'
+// descriptors and laser tables of one call, laid out as in the grid-owned
+// buffer of nm_sim_cast.  Each GPU holds its own copy; host code reaches
+// it only through the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[NM_SIM_STATIC_BYTES];
+
+namespace {
+
+constexpr int kStaticDevices = 64;
+std::mutex g_static_mu;
+unsigned char* g_static_ptr[kStaticDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up on the
+// first call on that device and kept.
+int static_scratch(int device, unsigned char** out) {
+  if (device < 0 || device >= kStaticDevices)
+    return sim_fail(NM_ENODEV, "nm_sim_cast_static: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return sim_fail(NM_EHIP, std::string("nm_sim_cast_static: "
+                                           "hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+    g_static_ptr[device] = (unsigned char*)p;
+  }
+  *out = g_static_ptr[device];
+  return NM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int nm_simgrid_from_leaves(int device, const int32_t* origins,
@@ -664,6 +698,44 @@ int nm_sim_cast(nm_simgrid* g, const nm_sim_sensor* sensors, int n_sensors,
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess)
     return sim_fail(NM_EHIP, std::string("nm_sim_cast: ") + hipGetErrorString(e));
+  return NM_OK;
+}
+
+int nm_sim_cast_static(nm_simgrid* g, const nm_sim_sensor* sensors,
+                       int n_sensors, int32_t* ijk, float* dist, float* xyz,
+                       int* returns, void* stream) {
+  if (!g || !sensors || n_sensors <= 0 || !returns)
+    return sim_fail(NM_EINVAL, "nm_sim_cast_static: bad argument");
+  if (hipSetDevice(g->device) != hipSuccess)
+    return sim_fail(NM_ENODEV, "nm_sim_cast_static: bad device");
+  size_t bytes = 0;
+  int st = nm::sim_setup(sensors, n_sensors, nullptr, 0, &bytes, nullptr);
+  if (st != NM_OK) return st;
+  if (nm::kSimImageOffset + bytes > (size_t)NM_SIM_STATIC_BYTES)
+    return sim_fail(NM_EINVAL,
+                    "nm_sim_cast_static: the sensors need " +
+                        std::to_string(nm::kSimImageOffset + bytes) +
+                        " bytes of descriptors, more than the fixed " +
+                        std::to_string(NM_SIM_STATIC_BYTES));
+  // host image of the inputs: a fixed per-thread array (each call
+  // synchronizes before it returns, so the next call may reuse it)
+  alignas(16) static thread_local unsigned char img[NM_SIM_STATIC_BYTES];
+  nm::SimLaunch L;
+  st = nm::sim_setup(sensors, n_sensors, img, sizeof(img), &bytes, &L);
+  if (st != NM_OK) return st;
+  unsigned char* dev = nullptr;
+  st = static_scratch(g->device, &dev);
+  if (st != NM_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(dev + nm::kSimImageOffset, img, bytes,
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = nm::sim_launch(g, L, dev, ijk, dist, xyz, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(returns, dev, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return sim_fail(NM_EHIP, std::string("nm_sim_cast_static: ") +
+                                 hipGetErrorString(e));
   return NM_OK;
 }
 
