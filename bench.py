@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=100,
                     help="length of the frame sequence")
+    ap.add_argument("--filter-frames", type=int, default=20,
+                    help="frames of the voxel-filter legs (0 = skip)")
     ap.add_argument("--cpu-frames", type=int, default=30,
                     help="frames of the CPU baseline sample (0 = skip)")
     ap.add_argument("--rounding", choices=["down", "nearest"], default="down")
@@ -292,6 +294,64 @@ def bench_soft(nm, W, torch, steps):
     }
 
 
+def bench_filter(nm, W, torch, ctx, frames, dev, P, cam, nf, rounding,
+               stream, sh):
+    """The first nf frames through the voxel filter, in each precision mode."""
+    # the same frames through the voxel filter (filterCloud precision 0):
+    # points outside the sensor's frustum box dropped, the rest binned per
+    # voxel, one ray per occupied voxel, into a second map
+    box = W.hall_box(nm, rounding)
+    fparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width, cam.height,
+                       nm.FILTER_VOXEL, box=box) for p, q, _ in frames[:nf]]
+    fmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
+    for k in range(2):
+        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    fa, fb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fa.record(stream)
+    for k in range(nf):
+        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+    fb.record(stream)
+    torch.cuda.synchronize()
+    ctx.set_timing(False)
+    fst = ctx.stage_times()
+    filtered = {
+        "workload": "the first %d frames of the sequence with the voxel "
+                    "filter: one ray per occupied 0.05 m voxel" % nf,
+        "frames": nf,
+        "ms_per_frame": fa.elapsed_time(fb) / nf,
+        "rays_per_frame": ctx.stats()["rays"],
+        "stage_ms": {k: round(v / n, 5) if n else 0.0
+                     for k, (v, n) in fst.items()},
+    }
+    fmap.close()
+    # the lower-memory variants (PrecisionType 1-3): FP16 sums and count,
+    # voxel centre with an int count, voxel centre with an 8-bit count
+    modes = {}
+    for name, filt in (("fp16_centroid", nm.FILTER_VOXEL_HALF),
+                       ("centre_int_count", nm.FILTER_VOXEL_COUNT),
+                       ("centre_8bit_count", nm.FILTER_VOXEL_COUNT8)):
+        mparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width,
+                           cam.height, filt, box=box)
+                   for p, q, _ in frames[:nf]]
+        mmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
+        for k in range(2):
+            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+        torch.cuda.synchronize()
+        fa.record(stream)
+        for k in range(nf):
+            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
+        fb.record(stream)
+        torch.cuda.synchronize()
+        modes[name] = {"ms_per_frame": round(fa.elapsed_time(fb) / nf, 4),
+                       "rays_last_frame": ctx.stats()["rays"]}
+        mmap.close()
+    filtered["precision_modes"] = modes
+    return filtered
+
+
 def cpu_baseline(frames_params, clouds, n):
     """Oracle (C++ restatement of Handler::processPointCloudCPU) on one
     pinned core: valid rays per second over the first n frames."""
@@ -435,6 +495,8 @@ def main():
     #  - clouds resident in HBM: event pair around each frame's submit;
     #  - as the baseline defines it, from the pinned host cloud to the map
     #    updated on the device: the event pair also spans the H2D copy.
+    dcam = nm.depth_camera(cam.width, cam.height, cam.hfov_deg, cam.vfov_deg)
+
     def sequence_latency(host):
         lat = []
         for k in range(3):
@@ -445,8 +507,11 @@ def main():
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            if host:
+            if host == "cloud":
                 ctx.submit(gmap, params[k], pinned[k].numpy(), stream=sh)
+            elif host == "depth":
+                ctx.submit_depth(gmap, params[k], dcam, pinned_depth[k].numpy(),
+                                 stream=sh)
             else:
                 submit(k)
             b.record(stream)
@@ -457,59 +522,15 @@ def main():
         return lat
 
     pinned = [torch.from_numpy(c).pin_memory() for c in clouds]
-    lat = sequence_latency(False)
-    pl = sequence_latency(True)
+    pinned_depth = [torch.from_numpy(np.ascontiguousarray(c[:, 2])).pin_memory()
+                    for c in clouds]
+    lat = sequence_latency(None)
+    pl = sequence_latency("cloud")
+    pd = sequence_latency("depth")
 
-    # the same frames through the voxel filter (filterCloud precision 0):
-    # points binned per voxel, one ray per occupied voxel, into a second map
-    nf = min(20, args.frames)
-    fparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width, cam.height,
-                       nm.FILTER_VOXEL) for p, q, _ in frames[:nf]]
-    fmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
-    for k in range(2):
-        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
-    torch.cuda.synchronize()
-    ctx.reset_timing()
-    ctx.set_timing(True)
-    fa, fb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    fa.record(stream)
-    for k in range(nf):
-        ctx.submit(fmap, fparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
-    fb.record(stream)
-    torch.cuda.synchronize()
-    ctx.set_timing(False)
-    fst = ctx.stage_times()
-    filtered = {
-        "workload": "the first %d frames of the sequence with the voxel "
-                    "filter: one ray per occupied 0.05 m voxel" % nf,
-        "frames": nf,
-        "ms_per_frame": fa.elapsed_time(fb) / nf,
-        "rays_per_frame": ctx.stats()["rays"],
-        "stage_ms": {k: round(v / n, 5) if n else 0.0
-                     for k, (v, n) in fst.items()},
-    }
-    fmap.close()
-    # the lower-memory variants (PrecisionType 1-3): FP16 sums and count,
-    # voxel centre with an int count, voxel centre with an 8-bit count
-    modes = {}
-    for name, filt in (("fp16_centroid", nm.FILTER_VOXEL_HALF),
-                       ("centre_int_count", nm.FILTER_VOXEL_COUNT),
-                       ("centre_8bit_count", nm.FILTER_VOXEL_COUNT8)):
-        mparams = [W.frame(nm, p, q, 0.05, 10.0, rounding, cam.width,
-                           cam.height, filt) for p, q, _ in frames[:nf]]
-        mmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 19)
-        for k in range(2):
-            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
-        torch.cuda.synchronize()
-        fa.record(stream)
-        for k in range(nf):
-            ctx.submit(mmap, mparams[k], (dev[k].data_ptr(), P, 12), stream=sh)
-        fb.record(stream)
-        torch.cuda.synchronize()
-        modes[name] = {"ms_per_frame": round(fa.elapsed_time(fb) / nf, 4),
-                       "rays_last_frame": ctx.stats()["rays"]}
-        mmap.close()
-    filtered["precision_modes"] = modes
+    nf = min(args.filter_frames, args.frames)
+    filtered = (bench_filter(nm, W, torch, ctx, frames, dev, P, cam, nf,
+                             rounding, stream, sh) if nf > 0 else None)
 
     # roofline of the dominant kernel stage: its duration over the timed
     # steps; the other stages from the untimed split
@@ -568,6 +589,11 @@ def main():
                               "warm-up frames" % len(pl),
         "p50_frame_ms_resident": percentile(lat, 50),
         "p99_frame_ms_resident": percentile(lat, 99),
+        "p50_frame_ms_depth_image": percentile(pd, 50),
+        "p99_frame_ms_depth_image": percentile(pd, 99),
+        "depth_image_note": "the same frames uploaded as pinned FP32 z-depth "
+                            "images (1.2 MB instead of 3.7 MB of xyz), "
+                            "back-projected in the cast's ray set-up",
         "voxel_updates_per_s": visits / dt,
         "rays_per_frame": R,
         "visits_per_frame": visits / K,

@@ -46,8 +46,10 @@ typedef struct nm_batch_tuning {
   int layout;    /* pixel tile: 0 = 16x16, 1 = 32x8, 2 = 64x4, 3 = 8x32     */
   int queue;     /* 0: one workgroup per tile; 1: persistent workgroups      */
                  /* draining a device queue in frame order; 2: the same,     */
-                 /* longest-range frames first                               */
-  int grid_mult; /* persistent workgroups per compute unit (queue 1, 2)      */
+                 /* longest-range frames first; 3: as 2, and the voxel pass  */
+                 /* takes the tiles in order of their longest ray (measured  */
+                 /* by the leaf pass), longest first                         */
+  int grid_mult; /* persistent workgroups per compute unit (queue 1-3)       */
   int lds_slots; /* LDS aggregation table of the accumulate pass: 4096 or   */
                  /* 2048 entries (fewer entries, more resident workgroups)  */
 } nm_batch_tuning;

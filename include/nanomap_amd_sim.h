@@ -25,6 +25,8 @@ typedef struct nm_simgrid nm_simgrid; /* device occupancy grid (5-4-3 tree) */
 
 #define NM_SENSOR_FRUSTUM 0 /* pinhole depth camera (SharedParameters _type 0) */
 #define NM_SENSOR_LASER 1   /* spherical scanner (_type 1)                   */
+#define NM_SENSOR_RAYS 2    /* explicit ray directions (an agent's
+                               observation sphere, Agent.h:186-195)       */
 
 /* One simulated sensor.  rotation = normalized(q) * F (row-major).
  * Frustum: width x height pixels; hfov/vfov are independent (the reference
@@ -35,7 +37,11 @@ typedef struct nm_simgrid nm_simgrid; /* device occupancy grid (5-4-3 tree) */
  * width = (int)(hfov / angular step + 1)); beam (w, h) has yaw
  * u = -hfov/2 + w hfov/width and pitch v = -vfov/2 + h vfov/height and
  * direction (sin u cos v, sin v, cos u cos v) in the sensor frame
- * (Sensor.h:134-155); the range limits are distances. */
+ * (Sensor.h:134-155); the range limits are distances.
+ * Rays: `width` directions (3 floats each, host memory, read during the
+ * call) in the sensor frame, height 1; the range limits are distances.
+ * Laser and ray directions are normalised, rotated by `rotation` and
+ * normalised again, as the laser branch of getRay does. */
 typedef struct nm_sim_sensor {
   float position[3];
   float rotation[9];
@@ -43,7 +49,8 @@ typedef struct nm_sim_sensor {
   int width, height;
   float hfov_deg, vfov_deg;
   float min_range, max_range;
-  int type; /* NM_SENSOR_* */
+  int type;           /* NM_SENSOR_* */
+  const float* rays;  /* NM_SENSOR_RAYS: width x 3 floats */
 } nm_sim_sensor;
 
 /* Occupancy grid of the voxels whose value exceeds `threshold`, from host

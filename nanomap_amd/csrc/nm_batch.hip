@@ -83,6 +83,8 @@ struct nm_batch {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev = nullptr;
+  hipEvent_t early_ev = nullptr;
+  hipEvent_t prep_ev = nullptr;
   // last batch
   int last_n = 0;
   std::vector<nm_map*> last_maps;
@@ -125,6 +127,8 @@ void free_batch(nm_batch* b) {
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+  if (b->early_ev) (void)hipEventDestroy(b->early_ev);
+  if (b->prep_ev) (void)hipEventDestroy(b->prep_ev);
   nm::workspace_free(&b->ws);
   delete b;
 }
@@ -275,6 +279,8 @@ int build_layout(nm_batch* b, const nm_agent_frame* f, int n) {
   p.side = b->side;
   p.fork = b->fork_ev;
   p.join = b->join_ev;
+  p.early_fork = b->early_ev;
+  p.prepared = b->prep_ev;
   b->layout_valid = true;
   return NM_OK;
 }
@@ -313,7 +319,7 @@ void fill_staging(nm_batch* b, Staging& s, const nm_agent_frame* f, int n,
 // The launch sequence of one batch from staging s.  dry: cast only, then
 // clear the per-frame state without touching any map.
 int enqueue(nm_batch* b, Staging& s, hipStream_t q, bool dry) {
-  const nm::Pipeline& p = b->pipe;
+  nm::Pipeline& p = b->pipe;
   nm::Workspace& w = b->ws;
   BH(hipMemcpyAsync(w.jobs, s.jobs, sizeof(nm::Job) * p.njobs,
                     hipMemcpyHostToDevice, q));
@@ -322,7 +328,10 @@ int enqueue(nm_batch* b, Staging& s, hipStream_t q, bool dry) {
                       hipMemcpyHostToDevice, q));
   BH(nm::launch_begin(w, nullptr, q));
   BH(nm::launch_discover(p, w, q));
+  BH(nm::launch_item_order(p, w, q));
   BH(nm::launch_compact_leaves(p, w, q));
+  p.map_prepared = false;
+  if (!dry) BH(nm::launch_map_prepare(p, w, w.maps, nm::MapView{}, q));
   BH(nm::launch_hit_keys(p, w, q));
   int sorted = 0;
   BH(nm::radix_sort_pairs(w.keys, w.vals, (int)p.n_rays, p.sort_bits,
@@ -418,6 +427,10 @@ int nm_batch_create(const nm_batch_config* cfg, nm_batch** out) {
     e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess)
     e = hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&b->early_ev, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&b->prep_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     free_batch(b);
     return bfail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
@@ -525,7 +538,7 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
       t.lds_slots = lds;
       t.queue = 0;
       cand.push_back(t);
-      for (int queue = 1; queue <= 2; ++queue)
+      for (int queue = 1; queue <= 3; ++queue)
         for (int gm : {2, 4, 6, 8}) {
           t.queue = queue;
           t.grid_mult = gm;
@@ -604,7 +617,7 @@ int nm_batch_tune(nm_batch* b, const nm_agent_frame* f, int n, int reps,
 int nm_batch_set_tuning(nm_batch* b, const nm_batch_tuning* t) {
   if (!b || !t) return bfail(NM_EINVAL, "nm_batch_set_tuning: null argument");
   if (t->layout < 0 || t->layout >= nm::kLayoutCount || t->queue < 0 ||
-      t->queue > 2 || t->grid_mult < 1 || t->grid_mult > 64 ||
+      t->queue > 3 || t->grid_mult < 1 || t->grid_mult > 64 ||
       (t->lds_slots != 4096 && t->lds_slots != 2048))
     return bfail(NM_EINVAL, "nm_batch_set_tuning: bad tuning");
   b->tuning.layout = t->layout;

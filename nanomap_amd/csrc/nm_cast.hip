@@ -284,10 +284,30 @@ __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
   return a;
 }
 
+// Largest value over the workgroup (256 threads).
+__device__ __forceinline__ uint32_t block_max(uint32_t v) {
+  __shared__ uint32_t s_max[4];
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
+// Sort key of an item's cost (its longest ray, in voxel visits): longer
+// rays give smaller keys, so an ascending sort puts them first.
+constexpr int kCostBits = 16;
+__device__ __forceinline__ uint32_t cost_key(uint32_t visits) {
+  const uint32_t top = (1u << kCostBits) - 1;
+  return top - (visits < top ? visits : top);
+}
+
+// cost_keys/cost_vals (queue 3 only, else null): the item's cost key and
+// index, the input of the item-order sort.
 template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
     const Job* __restrict__ jobs, ItemQueue q, int32_t* __restrict__ leaf_slot,
-    uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part) {
+    uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part,
+    uint32_t* __restrict__ cost_keys, uint32_t* __restrict__ cost_vals) {
   int round = 0;
   for (;;) {
     const int item = take_item(q, round);
@@ -354,6 +374,13 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
     const uint64_t v[4] = {visits, (uint64_t)valid,
                            (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
     block_partials(part + (size_t)item * 4, v, 4);
+    if (cost_keys) {
+      const uint32_t longest = block_max(visits);
+      if (threadIdx.x == 0) {
+        cost_keys[item] = cost_key(longest);
+        cost_vals[item] = (uint32_t)item;
+      }
+    }
   }
 }
 
@@ -649,7 +676,7 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
       int entry = slot_of(J.cube, ls, px, py, pz);
       double L = r.L;
       if (M == kDown) fp64_round_down_begin(s, L);
-      while (dda_step<M, false, M == kDown>(s, L)) {
+      while (dda_step<M, 2, M == kDown>(s, L)) {
         // miss on (px, py, pz)
         if (entry < 0) {
           ++errors;
@@ -1425,7 +1452,9 @@ ItemQueue cast_queue(const Pipeline& p, const Workspace& b, int which,
   *grid = p.nitems;
   if (p.tuning.queue != 0) {
     q.head = &b.counters->queue[which];
-    if (p.tuning.queue == 2) q.order = b.item_order;
+    if (p.tuning.queue >= 2) q.order = b.item_order;
+    if (p.tuning.queue == 3 && which == 1 && p.accum_order)
+      q.order = p.accum_order;
     const int g = p.cu_count * (p.tuning.grid_mult > 0 ? p.tuning.grid_mult : 1);
     *grid = p.nitems < g ? p.nitems : g;
   }
@@ -1456,13 +1485,29 @@ hipError_t launch_discover(const Pipeline& p, const Workspace& b,
   if (p.nitems <= 0) return hipSuccess;
   int grid = 0;
   const ItemQueue q = cast_queue(p, b, 0, &grid);
+  uint32_t* ck = p.tuning.queue == 3 ? b.cost_keys[0] : nullptr;
+  uint32_t* cv = p.tuning.queue == 3 ? b.cost_vals[0] : nullptr;
   if (p.rounding == kDown)
-    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(b.jobs, q, b.leaf_slot,
-                                                    b.ray_hit, b.part_discover);
+    k_discover<kDown><<<grid, kCastThreads, 0, s>>>(
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv);
   else
     k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover);
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv);
   return hipGetLastError();
+}
+
+hipError_t launch_item_order(Pipeline& p, const Workspace& b, hipStream_t s) {
+  p.accum_order = nullptr;
+  if (p.tuning.queue != 3 || p.nitems <= 0) return hipSuccess;
+  uint32_t* keys[2] = {b.cost_keys[0], b.cost_keys[1]};
+  uint32_t* vals[2] = {b.cost_vals[0], b.cost_vals[1]};
+  int out = 0;
+  // stable: items of equal cost keep the discover pass's order
+  const hipError_t e = radix_sort_pairs(keys, vals, p.nitems, kCostBits,
+                                        b.radix_hist, &out, s);
+  if (e != hipSuccess) return e;
+  p.accum_order = reinterpret_cast<const int32_t*>(b.cost_vals[out]);
+  return hipSuccess;
 }
 
 hipError_t launch_compact_leaves(const Pipeline& p, const Workspace& b,
@@ -1530,6 +1575,46 @@ hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
   return hipGetLastError();
 }
 
+namespace {
+
+// The map lookup and, unless already valid, the miss-run tables.
+void map_lookup_and_runs(const Pipeline& p, const Workspace& b,
+                         const MapView* map_table, const MapView& single,
+                         hipStream_t s) {
+  k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
+                                   single, p.tile_cap, b.counters, b.map_idx,
+                                   b.map_new);
+  const dim3 rgrid(kMissRuns / 256, p.njobs);
+  if (!p.miss_runs_ready) {
+    if (p.rounding == kDown) {
+      k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
+                                               b.runs_key);
+      k_runs_mark<kDown><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
+    } else {
+      k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
+                                                  b.runs_key);
+      k_runs_mark<kNearest><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_map_prepare(Pipeline& p, const Workspace& b,
+                              const MapView* map_table, const MapView& single,
+                              hipStream_t s) {
+  p.map_prepared = false;
+  if (!p.side || !p.early_fork || !p.prepared) return hipErrorInvalidValue;
+  hipError_t e = hipEventRecord(p.early_fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(p.side, p.early_fork, 0);
+  if (e != hipSuccess) return e;
+  map_lookup_and_runs(p, b, map_table, single, p.side);
+  e = hipEventRecord(p.prepared, p.side);
+  if (e != hipSuccess) return e;
+  p.map_prepared = true;
+  return hipGetLastError();
+}
+
 hipError_t launch_update(const Pipeline& p, const Workspace& b,
                          int sorted_buf, const MapView* map_table,
                          const MapView& single, hipStream_t s) {
@@ -1561,20 +1646,12 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
     const hipError_t e = hipEventRecord(p.join, p.side);
     if (e != hipSuccess) return e;
   }
-  k_map_lookup<<<256, 256, 0, s>>>(b.jobs, b.slot_job, b.slot_leaf, map_table,
-                                   single, p.tile_cap, b.counters, b.map_idx,
-                                   b.map_new);
-  const dim3 rgrid(kMissRuns / 256, p.njobs);
-  if (!p.miss_runs_ready) {
-    if (down) {
-      k_miss_runs<kDown><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
-                                               b.runs_key);
-      k_runs_mark<kDown><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
-    } else {
-      k_miss_runs<kNearest><<<rgrid, 256, 0, s>>>(b.jobs, b.miss_runs,
-                                                  b.runs_key);
-      k_runs_mark<kNearest><<<1, 64, 0, s>>>(b.jobs, p.njobs, b.runs_key);
-    }
+  if (p.map_prepared) {
+    // the lookup and the miss-run tables ran beside the cast
+    const hipError_t e = hipStreamWaitEvent(s, p.prepared, 0);
+    if (e != hipSuccess) return e;
+  } else {
+    map_lookup_and_runs(p, b, map_table, single, s);
   }
   if (down)
     k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,

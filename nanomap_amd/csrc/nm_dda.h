@@ -108,24 +108,33 @@ NM_HD void dda_init(const double* eye, const RayDir& r, Dda& s) {
 // struct fields can otherwise become a load from a selected address, which
 // pins the state to scratch memory).
 //
-// Two equivalent forms of the axis choice, picked per kernel by
-// measurement (the discover pass runs faster with kMin, the accumulate
-// pass without it):
-//   kMin:  t = min(nx, ny, nz); the axis is z if nz is minimal, else y if
-//          ny is, else x (+0 and -0 compare equal here as in the rule);
-//   else:  x only if strictly smallest, else y if y < z, else z, with t
-//          and the increment selected by bit masks.
+// Three equivalent forms of the axis choice, picked per kernel by
+// measurement:
+//   kMin = 1:  t = min(nx, ny, nz); the axis is z if nz is minimal, else y
+//              if ny is, else x (+0 and -0 compare equal here as in the
+//              rule);
+//   kMin = 0:  x only if strictly smallest, else y if y < z, else z, with t
+//              and the increment selected by bit masks;
+//   kMin = 2:  the same rule with t and the increment chosen by nested
+//              selects (two v_cndmask per 32-bit half instead of three and
+//              an or).
 //
 // kHwDown: the caller has set the wave's FP64 rounding mode to toward -inf,
 // so the round-down sum is the hardware add (one instruction instead of the
 // emulated add_down_pos).
-template <int M, bool kMin = false, bool kHwDown = false>
+template <int M, int kMin = 0, bool kHwDown = false>
 NM_HD bool dda_step(Dda& s, double L) {
   const double nx = s.nx, ny = s.ny, nz = s.nz;
   const double ddx = s.dx, ddy = s.dy, ddz = s.dz;
   bool bx, by, bz;
   double t, dt;
-  if (kMin) {
+  if (kMin == 2) {
+    bx = (nx < ny) && (nx < nz);
+    by = !bx && (ny < nz);
+    bz = !bx && !by;
+    t = bx ? nx : (by ? ny : nz);
+    dt = bx ? ddx : (by ? ddy : ddz);
+  } else if (kMin == 1) {
     t = fmin_(nx, fmin_(ny, nz));
     bz = nz == t;
     by = !bz && ny == t;

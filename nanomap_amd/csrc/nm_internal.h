@@ -174,8 +174,11 @@ __device__ __forceinline__ void load_point(const Job& p, int i, float& x,
 struct Tuning {
   int layout = kLayout16x16;
   int queue = 0;      // 0: one workgroup per item; 1: persistent workgroups
-                      // draining a queue in job order; 2: same, costliest
-                      // items first
+                      // draining a queue in job order; 2: same, longest-
+                      // range jobs first; 3: as 2 for the discover pass,
+                      // and the accumulate pass takes the items in order
+                      // of their longest ray, longest first (measured by
+                      // the discover pass)
   int grid_mult = 4;  // persistent workgroups per compute unit
   int lds_slots = 4096;  // LDS aggregation table of the accumulate pass:
                          // 4096 or 2048 entries
@@ -207,6 +210,8 @@ struct Workspace {
   uint64_t* part_discover;  // [item_cap * 4] per-item counter sums
   int32_t* item_job;    // [item_cap]  job of each work item
   int32_t* item_order;  // [item_cap]  queue order of the work items
+  uint32_t* cost_keys[2];  // [item_cap]  item cost keys (queue 3) and the
+  uint32_t* cost_vals[2];  //             item indices sorted by them
   unsigned char* cloud; // staging for host clouds
   float* filter_rays;   // [ray_cap * 4]
   float* filter_tmp;    // [ray_cap * 4]
@@ -266,9 +271,17 @@ struct Pipeline {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr;
   hipEvent_t join = nullptr;
+  // launch_map_prepare ran the map lookup and the miss-run tables on the
+  // side stream (early_fork, then prepared recorded there): launch_update
+  // then only waits for `prepared` before the per-voxel sums
+  hipEvent_t early_fork = nullptr;
+  hipEvent_t prepared = nullptr;
+  bool map_prepared = false;
   int rounding;
   int cu_count;
   Tuning tuning;
+  // queue 3: the accumulate pass's item order (set by launch_item_costs)
+  const int32_t* accum_order = nullptr;
 };
 
 hipError_t launch_begin(const Workspace& b, const Job* job0, hipStream_t s);
@@ -282,6 +295,17 @@ hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
                                int sorted_buf, hipStream_t s);
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
                              int sorted_buf, hipStream_t s);
+// Queue 3: sorts the items by the cost keys the discover pass wrote and
+// points p.accum_order at the result.
+hipError_t launch_item_order(Pipeline& p, const Workspace& b, hipStream_t s);
+// The update's steps that need only the compacted leaves and the job
+// table -- the map lookup (leaf indices, new-leaf reservation) and the
+// miss-run tables -- on the side stream, forked from `s` after the leaf
+// compaction so they overlap the hit sort and the voxel pass.  Sets
+// p.map_prepared.  Needs p.side, p.early_fork and p.prepared.
+hipError_t launch_map_prepare(Pipeline& p, const Workspace& b,
+                              const MapView* map_table, const MapView& single,
+                              hipStream_t s);
 // map_table: device table indexed by Job::map, or null to use `single`.
 hipError_t launch_update(const Pipeline& p, const Workspace& b,
                          int sorted_buf, const MapView* map_table,

@@ -73,7 +73,8 @@ struct SimDesc {
   float res, min_range, max_range;
   int width, height;
   int type;        // NM_SENSOR_*
-  int tab_off;     // laser: (sin, cos) of the W yaws, then of the H pitches
+  int tab_off;     // laser: (sin, cos) of the W yaws, then of the H pitches;
+                   // rays: W directions (3 floats each)
   int tile_off;    // first global tile of this sensor
   int tiles_x;     // tiles per row
   int64_t pix_off; // first output pixel of this sensor
@@ -173,17 +174,23 @@ __global__ __launch_bounds__(256) void k_sim_cast(
   // gpu::Sensor::getRay.  md: the ray in the world frame, dd: the sensor
   // frame direction the simulated point lies along.
   float md[3], dd[3], min_t, max_t;
-  if (d.type == NM_SENSOR_LASER) {
-    // spherical branch (Sensor.h:134-155): yaw u, pitch v of the beam
-    // (lanes past the image edge read beam 0: they only join the ballot)
+  if (d.type != NM_SENSOR_FRUSTUM) {
+    // lanes past the image edge read beam 0: they only join the ballot
     const int wc = in_image ? w : 0, hc = in_image ? h : 0;
-    const float su = tables[d.tab_off + 2 * wc];
-    const float cu = tables[d.tab_off + 2 * wc + 1];
-    const int ph = d.tab_off + 2 * d.width + 2 * hc;
-    const float sv = tables[ph], cv = tables[ph + 1];
-    dd[0] = su * cv;
-    dd[1] = sv;
-    dd[2] = cu * cv;
+    if (d.type == NM_SENSOR_LASER) {
+      // spherical branch (Sensor.h:134-155): yaw u, pitch v of the beam
+      const float su = tables[d.tab_off + 2 * wc];
+      const float cu = tables[d.tab_off + 2 * wc + 1];
+      const int ph = d.tab_off + 2 * d.width + 2 * hc;
+      const float sv = tables[ph], cv = tables[ph + 1];
+      dd[0] = su * cv;
+      dd[1] = sv;
+      dd[2] = cu * cv;
+    } else {  // explicit directions
+      dd[0] = tables[d.tab_off + 3 * wc];
+      dd[1] = tables[d.tab_off + 3 * wc + 1];
+      dd[2] = tables[d.tab_off + 3 * wc + 2];
+    }
     const float n0 = norm3(dd);
 #pragma unroll
     for (int i = 0; i < 3; ++i) dd[i] = dd[i] / n0;
@@ -384,10 +391,13 @@ int sim_setup(const nm_sim_sensor* sensors, int n, unsigned char* img,
   for (int k = 0; k < n; ++k) {
     const nm_sim_sensor& in = sensors[k];
     if (in.width <= 0 || in.height <= 0 || !(in.grid_res > 0) ||
-        (in.type != NM_SENSOR_FRUSTUM && in.type != NM_SENSOR_LASER) ||
+        (in.type != NM_SENSOR_FRUSTUM && in.type != NM_SENSOR_LASER &&
+         in.type != NM_SENSOR_RAYS) ||
+        (in.type == NM_SENSOR_RAYS && (!in.rays || in.height != 1)) ||
         (int64_t)in.width * in.height > (1 << 30))
       return setup_fail(NM_EINVAL, "nm_sim_cast: bad sensor");
     if (in.type == NM_SENSOR_LASER) n_tab += 2 * ((size_t)in.width + in.height);
+    if (in.type == NM_SENSOR_RAYS) n_tab += 3 * (size_t)in.width;
   }
   const size_t tab_off = ((size_t)n * sizeof(SimDesc) + 15) & ~(size_t)15;
   *bytes = tab_off + n_tab * sizeof(float);
@@ -422,6 +432,9 @@ int sim_setup(const nm_sim_sensor* sensors, int n, unsigned char* img,
         d.I[0][i] = e0[i] * 2 * hw;
         d.I[1][i] = e1[i] * 2 * hh;
       }
+    } else if (in.type == NM_SENSOR_RAYS) {
+      d.tab_off = toff;
+      for (int i = 0; i < 3 * in.width; ++i) tab[toff++] = in.rays[i];
     } else {
       // Sensor::getRay, laser branch (Sensor.h:134-143; scale from
       // updateShared, Sensor.h:57-58): FP32 angles in degrees, scaled to

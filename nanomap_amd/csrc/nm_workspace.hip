@@ -50,6 +50,10 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.part_discover, I * 4));
   A(dalloc(&b.item_job, I));
   A(dalloc(&b.item_order, I));
+  A(dalloc(&b.cost_keys[0], I));
+  A(dalloc(&b.cost_keys[1], I));
+  A(dalloc(&b.cost_vals[0], I));
+  A(dalloc(&b.cost_vals[1], I));
   A(dalloc(&b.miss_runs, J * kMissRuns));
   A(dalloc(&b.runs_key, J));
   if (d.host_clouds) {
@@ -84,7 +88,9 @@ void workspace_free(Workspace* b) {
                   b->radix_hist,
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
                   b->jobs,       b->maps,        b->part_discover,
-                  b->item_job,   b->item_order,  b->cloud,      b->filter_rays,
+                  b->item_job,   b->item_order,  b->cost_keys[0],
+                  b->cost_keys[1], b->cost_vals[0], b->cost_vals[1],
+                  b->cloud,      b->filter_rays,
                   b->filter_tmp, b->flags,       b->miss_runs,
                   b->runs_key};
   for (void* p : ptrs)
