@@ -274,6 +274,15 @@ int nm_ctx_set_timing_mask(nm_ctx* ctx, unsigned mask);
 int nm_ctx_stage_times(nm_ctx* ctx, float* ms, int64_t* launches);
 int nm_ctx_reset_timing(nm_ctx* ctx);
 
+/* ---- device buffers ------------------------------------------------------
+ * For host code that holds no HIP headers: device memory for clouds the
+ * library fills and reads back (e.g. simulated clouds fed to
+ * nm_frame_submit with cloud_on_device = 1). */
+int nm_device_alloc(int device, int64_t bytes, void** out);
+int nm_device_free(int device, void* ptr);
+/* Synchronous device-to-host copy. */
+int nm_device_download(int device, void* dst, const void* src, int64_t bytes);
+
 /* ---- host-side helpers (no device work) ------------------------------- */
 /* fold(v, a, k): v <- round(v + a), k times, in `rounding` (FP32). */
 float nm_fold_host(float v, float a, uint32_t k, int rounding);

@@ -10,7 +10,7 @@ import ctypes
 
 import numpy as np
 
-from ._capi import (SENSOR_FRUSTUM, SENSOR_LASER,
+from ._capi import (SENSOR_FRUSTUM, SENSOR_LASER, SENSOR_RAYS,
                     FILTER_NONE, FILTER_VOXEL, FILTER_VOXEL_COUNT,
                     FILTER_VOXEL_COUNT8, FILTER_VOXEL_HALF, ROUND_DOWN,
                     ROUND_NEAREST,
@@ -482,12 +482,19 @@ def camera(width, height, hfov_deg, vfov_deg, max_range, noise_k=0.002,
 
 
 def sim_sensor(pos, rotation, grid_res, width, height, hfov_deg, vfov_deg,
-               min_range, max_range, laser=False):
-    """A simulated sensor: a frustum camera of width x height pixels, or
-    with laser=True a spherical scanner of width x height beams over
-    hfov x vfov degrees (include/nanomap_amd_sim.h)."""
+               min_range, max_range, laser=False, rays=None):
+    """A simulated sensor: a frustum camera of width x height pixels, with
+    laser=True a spherical scanner of width x height beams over hfov x vfov
+    degrees, or with rays=[n, 3] n explicit directions (width n, height 1)
+    (include/nanomap_amd_sim.h).  The sensor keeps a reference to rays."""
     s = SimSensor()
     s.type = SENSOR_LASER if laser else SENSOR_FRUSTUM
+    if rays is not None:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 3)
+        s.type = SENSOR_RAYS
+        s.rays = r.ctypes.data
+        s._rays = r
+        width, height = r.shape[0], 1
     for k in range(3):
         s.position[k] = float(pos[k])
     for k, x in enumerate(np.asarray(rotation, np.float32).reshape(9)):

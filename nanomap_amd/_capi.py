@@ -112,11 +112,12 @@ class SimSensor(ctypes.Structure):
                 ("width", ctypes.c_int), ("height", ctypes.c_int),
                 ("hfov_deg", ctypes.c_float), ("vfov_deg", ctypes.c_float),
                 ("min_range", ctypes.c_float), ("max_range", ctypes.c_float),
-                ("type", ctypes.c_int)]
+                ("type", ctypes.c_int), ("rays", ctypes.c_void_p)]
 
 
 SENSOR_FRUSTUM = 0
 SENSOR_LASER = 1
+SENSOR_RAYS = 2
 SIM_STATIC_BYTES = 65536  # NM_SIM_STATIC_BYTES
 
 
@@ -175,6 +176,9 @@ SIGNATURES = {
                                 _PU32]),
     "nm_filter_rays_download": (_I, [_P, _I, ctypes.POINTER(_I), _PF]),
     "nm_fold_host": (_F, [_F, _F, ctypes.c_uint32, _I]),
+    "nm_device_alloc": (_I, [_I, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "nm_device_free": (_I, [_I, _P]),
+    "nm_device_download": (_I, [_I, _P, _P, ctypes.c_int64]),
     "nm_ctx_set_timing": (_I, [_P, _I]),
     "nm_ctx_set_timing_mask": (_I, [_P, ctypes.c_uint]),
     "nm_ctx_stage_times": (_I, [_P, _PF, _PI64]),

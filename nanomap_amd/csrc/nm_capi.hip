@@ -194,6 +194,37 @@ const char* nm_last_error(void) { return nm::g_error.c_str(); }
 
 int nm_version(void) { return 1; }
 
+int nm_device_alloc(int device, int64_t bytes, void** out) {
+  if (!out || bytes <= 0) return fail(NM_EINVAL, "nm_device_alloc: bad argument");
+  *out = nullptr;
+  NM_HIP(hipSetDevice(device));
+  const hipError_t e = hipMalloc(out, (size_t)bytes);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(e == hipErrorOutOfMemory ? NM_ENOMEM : NM_EHIP,
+                std::string("nm_device_alloc: ") + hipGetErrorString(e));
+  }
+  return NM_OK;
+}
+
+int nm_device_free(int device, void* ptr) {
+  if (!ptr) return NM_OK;
+  NM_HIP(hipSetDevice(device));
+  NM_HIP(hipDeviceSynchronize());
+  NM_HIP(hipFree(ptr));
+  return NM_OK;
+}
+
+int nm_device_download(int device, void* dst, const void* src, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src)))
+    return fail(NM_EINVAL, "nm_device_download: bad argument");
+  if (bytes == 0) return NM_OK;
+  NM_HIP(hipSetDevice(device));
+  NM_HIP(hipDeviceSynchronize());
+  NM_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return NM_OK;
+}
+
 float nm_fold_host(float v, float a, uint32_t k, int rounding) {
   return rounding == NM_ROUND_DOWN ? nm::fold<nm::kDown>(v, a, k)
                                    : nm::fold<nm::kNearest>(v, a, k);

@@ -70,6 +70,11 @@ class Config {
       if (line == "Sensor:") {
         in >> file;
         files.push_back(file);
+      } else if (line == "Agent:") {  // Config.cpp:129-131
+        in >> file;
+        _agentFiles.push_back(file);
+      } else if (line == "SimGrid:") {  // SimManager.h:183-185
+        in >> _simGrid;
       } else if (line == "#endconfig") {
         break;
       }
@@ -123,6 +128,10 @@ class Config {
   }
 
   std::vector<std::shared_ptr<sensor::SensorData>>& sensorData() { return _sensorData; }
+  // Agent config files ("Agent:" lines) and the simulation grid ("SimGrid:",
+  // here a map snapshot written by nm_map_save).
+  const std::vector<std::string>& agentFiles() const { return _agentFiles; }
+  const std::string& simGrid() const { return _simGrid; }
   std::shared_ptr<sensor::SensorData> sensorData(int index, bool searchById = false) {
     if (searchById) {
       for (auto& s : _sensorData)
@@ -171,6 +180,8 @@ class Config {
   int _frustumPCLSize = 0;
   int _laserPCLSize = 0;
   std::vector<std::shared_ptr<sensor::SensorData>> _sensorData;
+  std::vector<std::string> _agentFiles;
+  std::string _simGrid;
 };
 
 }  // namespace config

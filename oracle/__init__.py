@@ -45,8 +45,8 @@ SIGNATURES = {
     "or_simgrid_build": (_P, [_PI32, _PF, ctypes.c_int64, _F]),
     "or_simgrid_destroy": (None, [_P]),
     "or_simgrid_active_count": (ctypes.c_int64, [_P]),
-    "or_sim_cast": (_I, [_P, _PF, _PF, _F, _I, _I, _F, _F, _F, _F, _I, _I,
-                         _PI32, _PF, _PF]),
+    "or_sim_cast": (_I, [_P, _PF, _PF, _F, _I, _I, _F, _F, _F, _F, _I, _PF,
+                         _I, _PI32, _PF, _PF]),
     "or_soft_forward": (None, [_PD, _I, _I, _I, _PD, _D, _D, _PD, _I, _I, _I,
                                _D, _D, _D, _I, _PD]),
     "or_soft_backward": (None, [_PD, _I, _I, _I, _PD, _D, _D, _PD, _I, _I,
@@ -244,9 +244,11 @@ class SimGrid:
         return int(lib().or_simgrid_active_count(self.h))
 
     def cast(self, pos, R, res, width, height, hfov, vfov, min_range,
-             max_range, hierarchical=True, laser=False):
-        """(ijk [P,3], dist [P], xyz [P,3], returns) of one frustum, or of
-        one spherical laser scanner (width x height beams)."""
+             max_range, hierarchical=True, laser=False, rays=None):
+        """(ijk [P,3], dist [P], xyz [P,3], returns) of one frustum, of one
+        spherical laser scanner (width x height beams), or of explicit ray
+        directions (rays [width, 3], height 1)."""
+        rr = None if rays is None else np.ascontiguousarray(rays, np.float32)
         P = width * height
         ijk = np.zeros((P, 3), np.int32)
         dist = np.zeros(P, np.float32)
@@ -256,8 +258,9 @@ class SimGrid:
         n = lib().or_sim_cast(self.h, _p(p, ctypes.c_float),
                               _p(Rm, ctypes.c_float), res, width, height,
                               hfov, vfov, min_range, max_range,
-                              int(bool(laser)), int(hierarchical),
-                              _p(ijk, ctypes.c_int32),
+                              2 if rays is not None else int(bool(laser)),
+                              None if rays is None else _p(rr, ctypes.c_float),
+                              int(hierarchical), _p(ijk, ctypes.c_int32),
                               _p(dist, ctypes.c_float), _p(xyz, ctypes.c_float))
         return ijk, dist, xyz, n
 

@@ -754,8 +754,10 @@ int64_t or_simgrid_active_count(const or_simgrid* g) { return g->active; }
 
 int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float res, int width, int height, float hfov_deg,
-                float vfov_deg, float min_range, float max_range, int laser,
-                int hierarchical, int32_t* ijk, float* dist, float* xyz) {
+                float vfov_deg, float min_range, float max_range, int type,
+                const float* rays, int hierarchical, int32_t* ijk,
+                float* dist, float* xyz) {
+  const bool laser = type == 1;
   RoundGuard guard(0);
   // Laser beams (Sensor.h:134-143, scale from updateShared 57-58): yaw and
   // pitch in FP32 degrees, -fov/2 + i fov/n, scaled to radians through
@@ -799,12 +801,20 @@ int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
       auto len = [](const float* x) {
         return std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
       };
-      if (laser) {
+      if (type != 0) {
         // Sensor::getRay, laser branch (Sensor.h:134-155): the beam
-        // (sin u cos v, sin v, cos u cos v), normalised; the world ray is
-        // R times it, normalised; the point direction is normalised again
-        const float su = ls[w], cu = lc[w], sv = ls[width + h], cv = lc[width + h];
-        float b[3] = {su * cv, sv, cu * cv};
+        // (sin u cos v, sin v, cos u cos v), or a given direction, is
+        // normalised; the world ray is R times it, normalised; the point
+        // direction is normalised again
+        float b[3];
+        if (laser) {
+          const float su = ls[w], cu = lc[w], sv = ls[width + h], cv = lc[width + h];
+          b[0] = su * cv;
+          b[1] = sv;
+          b[2] = cu * cv;
+        } else {
+          for (int i = 0; i < 3; ++i) b[i] = rays[3 * (size_t)w + i];
+        }
         const float nb = len(b);
         for (int i = 0; i < 3; ++i) b[i] = b[i] / nb;
         for (int i = 0; i < 3; ++i)

@@ -97,16 +97,18 @@ or_simgrid* or_simgrid_build(const int32_t* origins, const float* values,
                              int64_t n_leaves, float threshold);
 void or_simgrid_destroy(or_simgrid* g);
 int64_t or_simgrid_active_count(const or_simgrid* g);
-/* Cast a W x H frustum (laser = 0) or a W x H-beam spherical scanner
- * (laser = 1).  Per pixel (row-major): first active voxel ijk (INT32_MIN if
+/* Cast a W x H frustum (type 0), a W x H-beam spherical scanner (type 1)
+ * or W explicit ray directions (type 2, rays: W x 3 floats, H = 1).  Per
+ * pixel (row-major): first active voxel ijk (INT32_MIN if
  * none in range), hit distance t*res in metres (-1 if none), and the
  * simulated sensor-frame point (NaN if no return).  hierarchical:
  * 1 = NanoVDB HDDA with node dims 1/8/128/4096, 0 = plain dim-1 DDA.
  * Returns the number of returns. */
 int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float res, int width, int height, float hfov_deg,
-                float vfov_deg, float min_range, float max_range, int laser,
-                int hierarchical, int32_t* ijk, float* dist, float* xyz);
+                float vfov_deg, float min_range, float max_range, int type,
+                const float* rays, int hierarchical, int32_t* ijk,
+                float* dist, float* xyz);
 
 /* ---- soft expected-depth march (include/nanomap_amd_soft.h), FP64 ------
  * grid: dense [nx][ny][nz]; cams: n_cams x 12 doubles (row-major optical ->

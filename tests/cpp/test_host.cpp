@@ -12,6 +12,9 @@
 //       Manager::insertPointCloud and writes the map's leaves to <out>
 //       (int64 count, then origins int32 [n,3], values float32 [n,512],
 //       masks uint32 [n,16]).
+//   test_host sim <config> <rounds>
+//       SimManager over the config's agents against a loop of Manager
+//       calls on the same simulated clouds (see sim() below).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -19,7 +22,12 @@
 #include <sstream>
 #include <vector>
 
+#include <cstring>
+#include <map>
+#include <tuple>
+
 #include "nanomap/manager/Manager.h"
+#include "nanomap/manager/SimManager.h"
 
 static int parse(const char* file) {
   auto c = std::make_shared<nanomap::config::Config>(file);
@@ -96,10 +104,102 @@ static int map(int argc, char** argv) {
   return out ? 0 : 3;
 }
 
+static bool same_maps(const std::shared_ptr<nanomap::map::Map>& a,
+                      const std::shared_ptr<nanomap::map::Map>& b) {
+  std::vector<int32_t> oa, ob;
+  std::vector<float> va, vb;
+  std::vector<uint32_t> ma, mb;
+  a->leaves(oa, va, ma);
+  b->leaves(ob, vb, mb);
+  if (oa.size() != ob.size()) return false;
+  // leaf order may differ: compare leaf by leaf through the origins
+  std::map<std::tuple<int, int, int>, size_t> idx;
+  for (size_t l = 0; l < ob.size() / 3; ++l)
+    idx[std::make_tuple(ob[3 * l], ob[3 * l + 1], ob[3 * l + 2])] = l;
+  for (size_t l = 0; l < oa.size() / 3; ++l) {
+    auto it = idx.find(std::make_tuple(oa[3 * l], oa[3 * l + 1], oa[3 * l + 2]));
+    if (it == idx.end()) return false;
+    if (std::memcmp(&va[512 * l], &vb[512 * it->second], 2048) != 0) return false;
+    if (std::memcmp(&ma[16 * l], &mb[16 * it->second], 64) != 0) return false;
+  }
+  return true;
+}
+
+// test_host sim <config> <rounds>: SimManager over the config's agents for
+// <rounds> rounds (every agent moved 0.1 m along x between rounds), then
+// the same simulated clouds through a loop of Manager::insertPointCloud
+// into fresh maps; prints the per-agent leaf counts and whether every map
+// is identical, then collision and spawn queries.
+static int sim(int argc, char** argv) {
+  auto cfg = std::make_shared<nanomap::config::Config>(argv[2]);
+  const int rounds = std::atoi(argv[3]);
+  nanomap::manager::SimManager sm(cfg);
+  nanomap::manager::Manager loop(cfg);
+  const size_t n = sm.agents().size();
+  std::vector<std::shared_ptr<nanomap::map::Map>> ref;
+  for (size_t i = 0; i < n; ++i)
+    ref.push_back(std::make_shared<nanomap::map::Map>(
+        cfg->mappingRes(), cfg->probHitThres(), cfg->probMissThres()));
+  int64_t returns = 0;
+  for (int r = 0; r < rounds; ++r) {
+    if (r > 0) {
+      std::vector<nanomap::Pose> poses = sm.agentPoses();
+      for (auto& p : poses) p.position(0) += 0.1f;
+      sm.updateAgentPoses(poses);
+    }
+    sm.updateAgentViews();
+    for (size_t i = 0; i < n; ++i) {
+      auto& a = sm.agents()[i];
+      for (size_t j = 0; j < a->sensorNames().size(); ++j) {
+        auto s = cfg->sensorData(a->sensorNames()[j]);
+        const auto& sh = s->sharedParameters();
+        std::vector<float> cloud = sm.lastCloud((int)i, (int)j);
+        for (size_t k = 0; k < cloud.size(); k += 3)
+          returns += cloud[k] == cloud[k] ? 1 : 0;
+        loop.insertPointCloud(a->sensorNames()[j], sh._hRes, sh._vRes, 12,
+                              reinterpret_cast<unsigned char*>(cloud.data()),
+                              a->sensorPose((int)j), ref[i]);
+      }
+    }
+  }
+  bool all = true;
+  int64_t leaves = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const bool same = same_maps(sm.getMap((int)i), ref[i]);
+    all = all && same;
+    leaves += sm.getMap((int)i)->leafCount();
+    if (!same) std::printf("agent %zu differs\n", i);
+  }
+  std::printf("agents %zu rounds %d returns %lld leaves %lld identical %d\n", n,
+              rounds, (long long)returns, (long long)leaves, all ? 1 : 0);
+  // collisions: agent 0's observation rays (sphere directions), its pose,
+  // and the first of 200 moves of 0.05 k m along +x that collides (-1 if
+  // none); then a seeded spawn
+  auto& a0 = *sm.agents()[0];
+  std::printf("rays %d %a\n", a0.numObservations(),
+              a0.observationNorms()[0]);
+  for (int k = 0; k < a0.numObservations(); ++k)
+    std::printf("ray %a %a %a\n", a0.observationRays()[3 * k],
+                a0.observationRays()[3 * k + 1], a0.observationRays()[3 * k + 2]);
+  std::printf("pose0 %a %a %a\n", a0.pose().position(0), a0.pose().position(1),
+              a0.pose().position(2));
+  int first = -1;
+  for (int k = 0; k < 200 && first < 0; ++k)
+    if (!sm.getAgentSimCollision(0, nanomap::Vector3f(0.05f * k, 0, 0)))
+      first = k;
+  nanomap::Pose sp = sm.getRandomSpawn(7);
+  std::printf("collision_step %d\nspawn %a %a %a\n", first, sp.position(0),
+              sp.position(1), sp.position(2));
+  sm.closeHandler();
+  loop.closeHandler();
+  return all ? 0 : 4;
+}
+
 int main(int argc, char** argv) {
   try {
     if (argc >= 3 && std::string(argv[1]) == "parse") return parse(argv[2]);
     if (argc >= 7 && std::string(argv[1]) == "map") return map(argc, argv);
+    if (argc >= 4 && std::string(argv[1]) == "sim") return sim(argc, argv);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -211,3 +211,118 @@ def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
     assert np.array_equal(go, oo)
     assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
     assert np.array_equal(gm, omk)
+
+
+AGENT = """#agentconfig
+Spawn:
+  Type:
+{spawn}
+Observations:
+  NumObs:
+80
+  LenObs:
+5
+  CollisionDist:
+0.3
+Sensor:
+  Name:
+cam
+  Position:
+0 0 0
+  Orientation:
+1.0 0.0 0.0 0.0
+#endconfig
+"""
+
+
+def _agent_config(tmp, cfg_path, snapshot, spawns):
+    """Append SimGrid and Agent lines (one agent file per spawn) to a
+    config written by _write_config."""
+    lines = ["SimGrid:\n%s\n" % snapshot]
+    for i, sp in enumerate(spawns):
+        p = os.path.join(tmp, "agent%d.txt" % i)
+        with open(p, "w") as f:
+            f.write(AGENT.format(spawn="Random" if sp is None else
+                                 "fixed\n  Position:\n%r %r %r\n  Orientation:\n"
+                                 "0 0 0 0" % tuple(float(x) for x in sp)))
+        lines.append("Agent:\n%s\n" % p)
+    text = open(cfg_path).read().replace("#endconfig\n", "".join(lines) +
+                                         "#endconfig\n")
+    with open(cfg_path, "w") as f:
+        f.write(text)
+
+
+@pytest.mark.gpu
+def test_sim_manager_32_agents_match_a_manager_loop(gpu, driver, tmp_path):
+    # SimManager over 32 agents (16 fixed spawns along the hall walk, 16
+    # random), two rounds of views: every agent's map equals, bit for bit,
+    # the map of a loop of Manager::insertPointCloud calls on the same
+    # simulated clouds.  The collision scan of agent 0 agrees with the
+    # oracle's cast of its observation rays, and the seeded spawn is an
+    # observed-free voxel of the grid.
+    from nanomap_amd import workloads as W
+    nm = gpu
+    tmp = str(tmp_path)
+    _, cam, frames = W.hall_walk_frames(nm, n_frames=3)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    sim_map = nm.Map(0.05, *W.CLAMP, hit_thr=0.0, miss_thr=0.0)
+    for pos, quat, cloud in frames:
+        ctx.submit(sim_map, W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN),
+                   cloud)
+    snap = os.path.join(tmp, "hall.nmap")
+    sim_map.save(snap)
+    o, v, _ = sim_map.download()
+    sim_map.close()
+    cfg = _write_config(tmp, 0.1, 0, [
+        FRUSTUM.format(name="cam", id=1, w=160, h=120, vfov=45, rmax=5.0,
+                       ph=0.7, pm=0.4)])
+    with open(cfg) as f:
+        text = f.read()
+    with open(cfg, "w") as f:  # GridRes = the snapshot's voxel size
+        f.write(text.replace("GridRes:\n0.1\n", "GridRes:\n0.05\n"))
+    scene, _ = W.hall_scene(nm)
+    walk, _ = scene.walk(2, 18, 0.1, 2.0, 15.0, 0.3)  # the hall walk's poses
+    spawns = [None] + [walk[k] for k in range(3, 18)] + [None] * 16
+    _agent_config(tmp, cfg, snap, spawns)
+    print("SimManager: 32 agents, 2 rounds", flush=True)
+    r = subprocess.run([driver, "sim", cfg, "2"], capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout.split("\n")
+    summary = [l for l in out if l.startswith("agents ")][0].split()
+    assert summary[1] == "32" and summary[-1] == "1"
+    assert int(summary[5]) > 0 and int(summary[7]) > 0  # returns, leaves
+
+    osim = oracle.SimGrid(o, v, 0.0)
+    head = [l for l in out if l.startswith("rays ")][0].split()
+    n_obs, coll = int(head[1]), np.float32(float.fromhex(head[2]))
+    rays = np.array([[float.fromhex(x) for x in l.split()[1:]]
+                     for l in out if l.startswith("ray ")], np.float32)
+    assert rays.shape == (n_obs, 3) == (80, 3)
+    assert np.allclose(np.linalg.norm(rays, axis=1), 1.0, atol=1e-5)
+    pose0 = np.array([float.fromhex(x) for x in
+                      [l for l in out if l.startswith("pose0 ")][0].split()[1:]],
+                     np.float32)
+    first = int([l for l in out if l.startswith("collision_step ")][0].split()[1])
+    max_range = np.float32(coll * np.float32(0.05))
+    eye_R = np.eye(3, dtype=np.float32)
+
+    def hits(k):
+        eye = pose0 + np.array([np.float32(0.05) * np.float32(k), 0, 0],
+                               np.float32)
+        return osim.cast(eye, eye_R, 0.05, n_obs, 1, 0.0, 0.0, 0.0,
+                         float(max_range), rays=rays)[3]
+
+    last = first if first >= 0 else 200
+    # the moves the library found free are free for the oracle, and the
+    # first it found colliding collides
+    assert all(hits(k) == 0 for k in range(last))
+    if first >= 0:
+        assert hits(first) > 0
+
+    sp = np.array([float.fromhex(x) for x in
+                   [l for l in out if l.startswith("spawn ")][0].split()[1:]])
+    ijk = np.floor(sp / 0.05).astype(np.int32)
+    loaded = nm.Map.load(snap)
+    val, act = loaded.query(ijk.reshape(1, 3))
+    assert not act[0] and val[0] < 0.0
