@@ -91,20 +91,32 @@ def test_hall_ten_frames_bit_exact(gpu):
 
 
 @pytest.mark.timeout(900)
-def test_hall_hundred_frames_bit_exact(gpu):
+def test_hall_hundred_frames_bit_exact(gpu, tmp_path):
     """The 100-frame 640x480 hall sequence at 0.05 m; every
-    frame's counts and the final map bit-exact versus the CPU path."""
+    frame's counts and the final map bit-exact versus the CPU path.  The
+    map is also saved after frame 50 and restored into a new map on a
+    fresh context, which takes frames 50-99 and ends bit-exact too."""
     nm = gpu
     _, cam, frames = W.hall_walk_frames(nm, n_frames=100)
     ctx = nm.Context(cam.width * cam.height, 200.0)
     gmap = nm.Map(0.05, *W.CLAMP)
     omap = oracle.Map(*W.CLAMP)
-    for pos, quat, cloud in frames:
+    restored, ctx2 = None, None
+    for k, (pos, quat, cloud) in enumerate(frames):
+        if k == 50:
+            gmap.save(tmp_path / "hall50.nmap")
+            restored = nm.Map.load(tmp_path / "hall50.nmap")
+            ctx2 = nm.Context(cam.width * cam.height, 200.0)
         f = W.frame(nm, pos, quat, 0.05, 10.0, RD, cam.width, cam.height)
         _run_frame(nm, ctx, gmap, omap, f, cloud)
+        if restored is not None:
+            ctx2.submit(restored, f, cloud)
     st = ctx.stats()
     assert st["errors"] == 0 and st["overflow"] == 0
     _check_maps(gmap, omap)
+    st2 = ctx2.stats()
+    assert st2["errors"] == 0 and st2["overflow"] == 0
+    _check_maps(restored, omap)
 
 
 def test_edge_cases(gpu):

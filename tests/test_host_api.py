@@ -213,6 +213,59 @@ def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
     assert np.array_equal(gm, omk)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("filter_type", [0, 1])
+def test_manager_laser_maps_match_the_oracle(gpu, driver, tmp_path,
+                                             filter_type):
+    # A #laserconfig sensor (901 x 16 beams, 30 m) through
+    # Manager::insertPointCloud: the reference sends laser clouds down the
+    # CPU path, which casts every point whatever the FilterType
+    # (Handler.cpp:247-249, processPointCloudCPU), so the map is the
+    # oracle's cast of every point of the cloud.  Clouds are beam
+    # directions at ranges 0.3-34 m (some beyond the 30 m range, some NaN).
+    nm = gpu
+    res, rmax, w, h = 0.1, 30.0, 901, 16
+    cfg = _write_config(str(tmp_path), res, filter_type, [LASER])
+    rng = np.random.default_rng(11 + filter_type)
+    az = np.radians(np.arange(w) * 0.4 - 180.0)
+    el = np.radians(np.arange(h) * 2.0 - 15.0)
+    A, E = np.meshgrid(az, el)
+    dirs = np.stack([np.cos(E) * np.cos(A), np.cos(E) * np.sin(A), np.sin(E)],
+                    -1).reshape(-1, 3)
+    omap = oracle.Map(float(np.float32(math.log(0.12) - math.log(1 - 0.12))),
+                      float(np.float32(math.log(0.97) - math.log(1 - 0.97))),
+                      0.0, 0.0)
+    args = []
+    for k in range(3):
+        r = rng.uniform(0.3, 34.0, size=(w * h, 1))
+        cloud = (dirs * r).astype(np.float32)
+        cloud[rng.random(w * h) < 0.05] = np.nan
+        path = os.path.join(str(tmp_path), "scan%d.bin" % k)
+        cloud.tofile(path)
+        p = rng.uniform(-5.0, 5.0, size=3).astype(np.float32)
+        q = rng.normal(size=4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        args.append("%s,%r,%r,%r,%r,%r,%r,%r" % (path, *[float(x) for x in p],
+                                                 *[float(x) for x in q]))
+        f = nm.frame_from_pose(p, q, np.eye(3, dtype=np.float32), res, rmax,
+                               float(_logit(0.8)), float(_logit(0.35)),
+                               nm.ROUND_DOWN, w, h, nm.FILTER_NONE)
+        t = oracle.Temp()
+        t.cast_cloud(f, cloud)
+        omap.integrate(t, nm.ROUND_DOWN)
+    out = os.path.join(str(tmp_path), "map.bin")
+    print("host driver: 3 laser scans, filter", filter_type, flush=True)
+    r = subprocess.run([driver, "map", cfg, "lidar", str(w), str(h), out] +
+                       args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    go, gv, gm = _read_map(out)
+    oo, ov, omk = omap.export()
+    assert go.shape == oo.shape and len(go) > 1000
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
+    assert np.array_equal(gm, omk)
+
+
 AGENT = """#agentconfig
 Spawn:
   Type:
