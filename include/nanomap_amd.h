@@ -76,7 +76,7 @@ typedef struct nm_ctx_config {
 } nm_ctx_config;
 
 /* Per-frame parameters, precomputed on the host.  Replaces the device copy
- * of gpu::Sensor + gpu::GridData (include/nanomap/gpu/Sensor.h:156-211,
+ * of gpu::Sensor + gpu::GridData (include/nanomap/gpu/Sensor.h:189-211,
  * GridData.h:38-62; uploaded at SensorAllocator.h:120-121). */
 typedef struct nm_frame {
   float rotation[9];    /* row-major R = normalized(q) * F (Handler.cpp:41) */
@@ -155,7 +155,7 @@ int nm_map_params(nm_map* map, float* grid_res, float* clamp_min,
 /* Grows the map to hold at least leaf_capacity leaves (pool and hash),
  * keeping every leaf, value and mask.  Synchronous; a no-op when the map
  * already holds that many.  The reference's OpenVDB grid grows per leaf
- * (OccupancyMap.h:132); this pool grows on request or by auto-grow. */
+ * (OccupancyMap.h:44, 69); this pool grows on request or by auto-grow. */
 int nm_map_reserve(nm_map* map, int64_t leaf_capacity);
 /* Auto-grow (off by default): nm_update_map first waits for the cast and,
  * when the map's free leaves are fewer than the leaves the cast touched,

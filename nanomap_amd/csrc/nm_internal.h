@@ -44,7 +44,7 @@ struct JobStats {
 // The sparse map on the device: an open-addressing hash of packed leaf
 // coordinates -> leaf index, and a leaf pool of 512 FP32 log-odds plus a
 // 512-bit active mask per leaf.  Stands in for the openvdb::FloatGrid of
-// map::Map (include/nanomap/map/OccupancyMap.h:132).
+// map::Map (include/nanomap/map/OccupancyMap.h:44-51, 69).
 struct MapView {
   unsigned long long* hash_keys;  // [hash_cap], ~0 = empty
   int32_t* hash_vals;             // [hash_cap]

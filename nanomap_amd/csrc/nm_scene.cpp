@@ -11,7 +11,7 @@
 // seeded per pixel, so rendering is deterministic under threading.
 //
 // nm_frame_from_pose restates the host-side pose set-up of the reference:
-// R = normalized(q).toRotationMatrix() * F (Handler.cpp:41, Sensor.h:181-182),
+// R = normalized(q).toRotationMatrix() * F (Handler.cpp:41, Sensor.h:87-88),
 // eye = worldToIndex(position) (Handler.cpp:34), max_time = maxRange /
 // gridRes (Handler.cpp:39), evaluated in the frame's rounding mode.
 #include <math.h>

@@ -437,7 +437,7 @@ int sim_setup(const nm_sim_sensor* sensors, int n, unsigned char* img,
       for (int i = 0; i < 3 * in.width; ++i) tab[toff++] = in.rays[i];
     } else {
       // Sensor::getRay, laser branch (Sensor.h:134-143; scale from
-      // updateShared, Sensor.h:57-58): FP32 angles in degrees, scaled to
+      // updateLaser, Sensor.h:57-58): FP32 angles in degrees, scaled to
       // radians through double; sin and cos of the FP32 angle, correctly
       // rounded through double
       d.tab_off = toff;

@@ -759,7 +759,7 @@ int or_sim_cast(const or_simgrid* g, const float pos[3], const float R[9],
                 float* dist, float* xyz) {
   const bool laser = type == 1;
   RoundGuard guard(0);
-  // Laser beams (Sensor.h:134-143, scale from updateShared 57-58): yaw and
+  // Laser beams (Sensor.h:134-143, scale from updateLaser 57-58): yaw and
   // pitch in FP32 degrees, -fov/2 + i fov/n, scaled to radians through
   // double; sin and cos of the FP32 angle rounded from double.
   std::vector<float> ls, lc;  // W yaws then H pitches
