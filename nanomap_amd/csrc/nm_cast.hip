@@ -573,7 +573,10 @@ template <int S>
 __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
                                           uint32_t* tiles, uint32_t* gap,
                                           uint32_t cell, uint32_t inc) {
-  uint32_t h = (cell * 2654435761u) >> (32 - S);  // multiplicative hash
+  // multiplicative hash on 24 bits (a full-rate multiply): the key's bits
+  // from 24 up are folded in first; bits [32-S, 32) of the product depend on
+  // every bit of the folded key
+  uint32_t h = (uint32_t)__umul24(cell ^ (cell >> 12), 0x9e3779u) >> (32 - S);
   int found = -1;
 #pragma unroll
   for (int q = 0; q < kProbe; ++q) {
