@@ -631,9 +631,13 @@ __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
   const bool head = !left_act || left != cell;
   const uint64_t stop = __ballot(head) | ~act | 0x0101010101010101ull;
   if (head) {
-    const uint64_t after = lane == 63 ? 0ull : (stop >> (lane + 1));
-    const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u
-                               : (uint32_t)(64 - lane);
+    // a run ends at the next stop bit above the lane, which is at most 8
+    // lanes up (every row start is a stop), so 32 bits of the shifted mask
+    // suffice; past lane 63 there is none, which min(., 63 - lane) covers
+    // (lane 63's shift by 0 included)
+    const uint32_t after = (uint32_t)(stop >> ((lane + 1) & 63));
+    const uint32_t len =
+        min((uint32_t)__builtin_ctzg(after, 32), (uint32_t)(63 - lane)) + 1u;
     lds_count<S>(s_key, s_cnt, tiles, gap, cell, len);
   }
 }
