@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("Mrays/s + p50 frame latency, 640x480@0.05m; batched & "
           "fwd+adjoint vs plain")
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X: one wave64 VALU instruction per 4 cycles on each
+# of the 1024 SIMDs at 2.4 GHz, in wave-instructions per second
+VALU_PEAK = 1024 * 2.4e9 / 4
+# wave64 VALU instructions per voxel visit of the accumulate pass on this
+# frame workload, from its SQ counters (profiles/k_accumulate_sq_counters.md)
+VALU_PER_VISIT = 2.15
 
 
 def parse():
@@ -185,6 +191,48 @@ def bench_sim(nm, W, torch, steps):
         "mrays_per_s": rays / (ms * 1e-3) / 1e6,
         "active_voxels": grid.info()["active"],
         "steps": steps,
+        "dense": bench_sim_dense(nm, W, torch, steps),
+    }
+
+
+def bench_sim_dense(nm, W, torch, steps):
+    """The same 64 x 848x480 cast through a dense grid: the 0.05 m map of
+    the first 20 hall frames (log-odds > 0), from 64 poses of the hall walk,
+    so rays traverse mapped free space and end on mapped surfaces."""
+    scene, cam, frames = W.hall_walk_frames(nm, n_frames=20)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    m = nm.Map(0.05, *W.CLAMP)
+    for pos, quat, cloud in frames:
+        ctx.submit(m, W.frame(nm, pos, quat, 0.05, 10.0, nm.ROUND_DOWN), cloud)
+    o, v, _ = m.download()
+    grid = nm.SimGrid(o, v, 0.0)
+    w, h, res = 848, 480, 0.05
+    pos, quat = scene.walk(2, 64, 0.1, 2.0, 15.0, 0.3)
+    sensors = []
+    for k in range(64):
+        fr = nm.frame_from_pose(pos[k], quat[k], nm.DEFAULT_F, res, 15.0,
+                                0.85, -0.4, 0)
+        sensors.append(nm.sim_sensor(pos[k], list(fr.rotation), res, w, h,
+                                     90.0, 60.0, 0.1, 15.0))
+    n = 0
+    for _ in range(2):
+        _, _, _, n = grid.cast(sensors)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, _, _, n = grid.cast(sensors)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    rays = len(sensors) * w * h
+    return {
+        "workload": "64 hall-walk poses x 848x480 (90x60 deg, 15 m) through "
+                    "the 0.05 m map of 20 hall frames, log-odds > 0",
+        "rays": rays,
+        "returns": n,
+        "ms_per_pass": ms,
+        "mrays_per_s": rays / (ms * 1e-3) / 1e6,
+        "active_voxels": grid.info()["active"],
+        "leaves": int(len(o)),
     }
 
 
@@ -609,6 +657,19 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(dom),
             "alg_bytes_per_launch": alg_bytes[dom],
+        },
+        # the second bound of the dominant kernel: VALU issue, with the
+        # instructions per visit from its counters and the visits and
+        # duration of the timed launches
+        "roofline_issue": {
+            "kernel_stage": "accumulate",
+            "bound": "valu_issue",
+            "achieved": visits / K * VALU_PER_VISIT / (avg["accumulate"] * 1e-3),
+            "peak": VALU_PEAK,
+            "unit": "wave64 VALU instructions/s",
+            "frac": visits / K * VALU_PER_VISIT / (avg["accumulate"] * 1e-3)
+                    / VALU_PEAK,
+            "valu_per_visit": VALU_PER_VISIT,
         },
         "roofline_frame": {
             "alg_bytes": frame_bytes,

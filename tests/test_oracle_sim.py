@@ -227,3 +227,60 @@ def test_laser_hierarchical_matches_dim1(oracle_lib):
         total += len(same)
         assert na > 1000
     assert agree / total > 0.999
+
+
+def test_explicit_rays_closed_form(oracle_lib):
+    # explicit directions (type 2, the observation rays of SimManager's
+    # collision query): the world ray is R times the normalised direction,
+    # ranges are distances.  A wall at x index 30 seen from x = 0.55 m is at
+    # 3.0 - 0.55 m along +x; -x sees nothing; an unnormalised (2, 0, 0) is
+    # the same ray; with R taking sensor +z to world +x, (0, 0, 1) hits it too
+    g = _wall_grid(0, 30)
+    I = np.eye(3, dtype=np.float32)
+    rays = np.array([[1, 0, 0], [-1, 0, 0], [2, 0, 0]], np.float32)
+    ijk, d, xyz, n = g.cast([0.55, 0.05, 0.05], I, 0.1, 3, 1, 0.0, 0.0, 0.0,
+                            20.0, rays=rays)
+    assert tuple(ijk[0]) == (30, 0, 0) and abs(d[0] - 2.45) < 1e-5
+    assert d[1] == -1 and np.isnan(xyz[1]).all()
+    assert tuple(ijk[2]) == (30, 0, 0) and d[2] == d[0]
+    assert n == 2
+    assert np.allclose(xyz[0], [d[0], 0, 0], atol=1e-6)
+    R = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]], np.float32)
+    ijk, d, _, _ = g.cast([0.55, 0.05, 0.05], R, 0.1, 1, 1, 0.0, 0.0, 0.0,
+                          20.0, rays=np.array([[0, 0, 1]], np.float32))
+    assert tuple(ijk[0]) == (30, 0, 0) and abs(d[0] - 2.45) < 1e-5
+    # a range below the wall's distance: no return
+    _, d, _, n = g.cast([0.55, 0.05, 0.05], I, 0.1, 1, 1, 0.0, 0.0, 0.0, 2.0,
+                        rays=rays[:1])
+    assert d[0] == -1 and n == 0
+
+
+def test_explicit_rays_reproduce_the_laser(oracle_lib):
+    # the laser's beams (sin u cos v, sin v, cos u cos v), u = -180 + w deg,
+    # v = -15 + 1.875 h deg, given as explicit rays: the same first voxels
+    # over the hall map (directions from numpy, so equal up to the last bit
+    # of the beam tables)
+    import nanomap_amd as nm
+    g, frames = _hall_oracle_grid(2)
+    W, H = 360, 16
+    u = np.radians(-180.0 + np.arange(W) * 1.0)
+    v = np.radians(-15.0 + np.arange(H) * 1.875)
+    U, V = np.meshgrid(u, v)
+    beams = np.stack([np.sin(U) * np.cos(V), np.sin(V), np.cos(U) * np.cos(V)],
+                     -1).reshape(-1, 3).astype(np.float32)
+    agree = total = 0
+    for pos, quat, _ in frames:
+        fr = nm.frame_from_pose(pos, quat, nm.DEFAULT_F, 0.05, 20.0, 0.85,
+                                -0.4, 0)
+        R = np.array(list(fr.rotation), np.float32)
+        a_ijk, a_d, _, na = g.cast(pos, R, 0.05, W, H, 360.0, 30.0, 0.0,
+                                   20.0, laser=True)
+        b_ijk, b_d, _, nb = g.cast(pos, R, 0.05, W * H, 1, 0.0, 0.0, 0.0,
+                                   20.0, rays=beams)
+        same = np.all(a_ijk == b_ijk, axis=1)
+        agree += same.sum()
+        total += len(same)
+        hit = same & (a_d >= 0)
+        assert np.allclose(a_d[hit], b_d[hit], atol=1e-4)
+        assert na > 500
+    assert agree / total > 0.999
