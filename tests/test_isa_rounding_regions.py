@@ -126,6 +126,23 @@ def _mode_states(blocks):
             else [None] * len(b[1]) for k, b in enumerate(blocks)]
 
 
+# The rounded FP64 operations of each round-down region, counted for this
+# compiler (ROCm 7.2 hipcc, -O3).  The accumulate region is the voxel loop:
+# its one add is the crossing-time add c + delta, and everything else is a
+# compare.  The discover region holds the axis steps' adds (single rounded
+# steps, and the in-binade jumps t + 8 D, exact), the jump's exact scalings
+# by powers of two and its floor.  Any other count means the compiler moved
+# code into or out of a region (an emulated set-up add, say, would run in
+# the wrong mode and still pass the mnemonic check above): re-inspect the
+# region before changing the numbers.
+EXPECTED_RD_OPS = {
+    "accumulate": {"v_add_f64": 1, "v_mul_f64": 0, "v_floor_f64": 0,
+                   "v_min_f64": 0, "v_max_f64": 0},
+    "discover": {"v_add_f64": 14, "v_mul_f64": 3, "v_floor_f64": 1,
+                 "v_min_f64": 2, "v_max_f64": 5},
+}
+
+
 def test_round_down_regions_hold_only_intended_operations(cast_asm):
     funcs = _functions(cast_asm)
     seen = {}
@@ -138,6 +155,7 @@ def test_round_down_regions_hold_only_intended_operations(cast_asm):
         blocks = _blocks(body)
         states = _mode_states(blocks)
         rd_adds = 0
+        counts = {op: 0 for op in EXPECTED_RD_OPS.get(kind, {})}
         for b, st in zip(blocks, states):
             for t, m in zip(b[1], st):
                 op = t.split()[0]
@@ -155,9 +173,14 @@ def test_round_down_regions_hold_only_intended_operations(cast_asm):
                     if FP_OP.match(op):
                         assert ALLOWED_FP.match(op), (name, b[0], t)
                     rd_adds += op == "v_add_f64"
+                    base = re.sub(r"_e(32|64)$", "", op)
+                    if base in counts:
+                        counts[base] += 1
                 elif m == "rn":
                     assert not END.search(t), ("restore outside region", name, t)
         assert rd_adds >= 1, (name, "no crossing add in round-down mode")
+        if kind in EXPECTED_RD_OPS:
+            assert counts == EXPECTED_RD_OPS[kind], (name, counts)
     # both loops use the hardware mode in their round-down instantiations
     # (one k_discover, one k_accumulate per LDS table size)
     assert seen.get("discover", 0) >= 1 and seen.get("accumulate", 0) >= 2, seen
