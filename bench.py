@@ -262,7 +262,7 @@ def bench_soft(nm, W, torch, steps):
                                for k in sel])).cuda()
     march = soft.SoftMarch(grid.shape, lo * res, res, 320, 240, 87.0, 58.0,
                            res, 160)
-    g32 = torch.from_numpy(grid).cuda()
+    g32 = m.dense(lo, grid.shape)  # gathered on the device from the map
     cams = soft.cameras(R, p)
     n = cams.shape[0]
     gout = torch.ones((n, 240, 320), dtype=torch.float32, device="cuda")

@@ -173,6 +173,13 @@ int nm_map_set_auto_grow(nm_map* map, int enable);
  * query buffer, and the call returns when the results are in place. */
 int nm_map_query(nm_map* map, const int32_t* ijk, int64_t n, float* values,
                  uint8_t* active, int on_device, void* stream);
+/* Dense copy of a box of the map into device memory: out[(i * ny + j) * nz
+ * + k] = log-odds of voxel (lo[0] + i, lo[1] + j, lo[2] + k), `fill` where
+ * the voxel's leaf is absent; asynchronous on `stream`.  No counterpart in
+ * the reference: the dense grid the soft march reads (nanomap_amd_soft.h),
+ * built from the device map without a host round trip. */
+int nm_map_dense(nm_map* map, const int32_t lo[3], int nx, int ny, int nz,
+                 float fill, float* out, void* stream);
 
 /* Snapshot and restore.  nm_map_save writes the map to `path`: a 64-byte
  * header ("NMAPLEAF", format version 1, leaf count, voxel size, clamps,

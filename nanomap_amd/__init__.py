@@ -326,6 +326,17 @@ class Map:
                                  stream))
         return vals[:len(a)], act[:len(a)].astype(bool)
 
+    def dense(self, lo, shape, fill=0.0, stream=None):
+        """Device tensor [nx, ny, nz] float32 of the voxels lo + (i, j, k),
+        `fill` where the leaf is absent (nm_map_dense)."""
+        import torch
+        nx, ny, nz = (int(x) for x in shape)
+        out = torch.empty((nx, ny, nz), dtype=torch.float32, device="cuda")
+        l = (ctypes.c_int32 * 3)(*(int(x) for x in lo))
+        check(lib().nm_map_dense(self.handle, l, nx, ny, nz, float(fill),
+                                 ctypes.c_void_p(out.data_ptr()), stream))
+        return out
+
     def save(self, path):
         check(lib().nm_map_save(self.handle, str(path).encode()))
 

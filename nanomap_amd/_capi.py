@@ -157,6 +157,7 @@ SIGNATURES = {
     "nm_map_reserve": (_I, [_P, ctypes.c_int64]),
     "nm_map_set_auto_grow": (_I, [_P, _I]),
     "nm_map_query": (_I, [_P, _P, ctypes.c_int64, _P, _P, _I, _P]),
+    "nm_map_dense": (_I, [_P, _P, _I, _I, _I, ctypes.c_float, _P, _P]),
     "nm_map_save": (_I, [_P, ctypes.c_char_p]),
     "nm_map_load": (_I, [_I, ctypes.c_char_p, ctypes.c_int64,
                          ctypes.POINTER(_P)]),

@@ -84,6 +84,30 @@ __global__ __launch_bounds__(256) void k_map_query(
   }
 }
 
+// Dense copy of the box [x0, x0 + nx) x [y0, y0 + ny) x [z0, z0 + nz): out
+// [(i * ny + j) * nz + k] = value of voxel (x0 + i, y0 + j, z0 + k), or
+// `fill` where its leaf is absent.  Consecutive lanes take consecutive k,
+// so a wavefront's lookups hit one or two leaves.
+__global__ __launch_bounds__(256) void k_map_dense(MapView m, int x0, int y0,
+                                                   int z0, int nx, int ny,
+                                                   int nz, float fill,
+                                                   float* __restrict__ out) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * 256) {
+    const int k = (int)(t % nz);
+    const int64_t r = t / nz;
+    const int j = (int)(r % ny);
+    const int i = (int)(r / ny);
+    const int x = x0 + i, y = y0 + j, z = z0 + k;
+    const int leaf = map_find(m, x, y, z);
+    out[t] = leaf >= 0
+                 ? m.values[(size_t)leaf * 512 +
+                            (((x & 7) << 6) | ((y & 7) << 3) | (z & 7))]
+                 : fill;
+  }
+}
+
 template <class T>
 hipError_t dalloc(T** p, size_t count) {
   return hipMalloc((void**)p, count * sizeof(T) + 64);
@@ -272,6 +296,26 @@ int nm_map_query(nm_map* m, const int32_t* ijk, int64_t n, float* values,
   if (active)
     NM_HIP(hipMemcpyAsync(active, d_act, (size_t)n, hipMemcpyDeviceToHost, s));
   NM_HIP(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+int nm_map_dense(nm_map* m, const int32_t lo[3], int nx, int ny, int nz,
+                 float fill, float* out, void* stream) {
+  if (!m || !lo || !out || nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(NM_EINVAL, "nm_map_dense: bad argument");
+  // the box's voxels must have leaf keys (21 bits per axis around 0)
+  const int64_t lim = (int64_t)1 << 23;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t d = a == 0 ? nx : (a == 1 ? ny : nz);
+    if ((int64_t)lo[a] < -lim || (int64_t)lo[a] + d > lim)
+      return fail(NM_EINVAL, "nm_map_dense: box outside the map's key range");
+  }
+  NM_HIP(hipSetDevice(m->device));
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  nm::k_map_dense<<<grid, 256, 0, (hipStream_t)stream>>>(
+      m->v, lo[0], lo[1], lo[2], nx, ny, nz, fill, out);
+  NM_HIP(hipGetLastError());
   return NM_OK;
 }
 

@@ -38,14 +38,18 @@ def soft_walk(gpu):
     o, v, _ = m.download()
     grid, lo = W.dense_grid(o, v)
     print("soft_walk grid", grid.shape, "leaves", len(o), flush=True)
+    # the same grid gathered on the device from the map (nm_map_dense)
+    dev = m.dense(lo, grid.shape)
+    torch.cuda.synchronize()
+    assert np.array_equal(dev.cpu().numpy().view(np.uint32),
+                          grid.view(np.uint32))
     sel = list(range(0, 48, 3))[:16]
     R = torch.tensor(np.stack([np.array(list(frames[k].rotation),
                                         np.float64).reshape(3, 3) for k in sel]))
     p = torch.tensor(np.stack([np.asarray(pos[k], np.float64) for k in sel]))
     march = soft.SoftMarch(grid.shape, lo * RES, RES, 320, 240, 87.0, 58.0,
                            RES, 160)
-    g32 = torch.from_numpy(grid).cuda()
-    return nm, march, g32, R.cuda(), p.cuda(), grid, lo
+    return nm, march, dev, R.cuda(), p.cuda(), grid, lo
 
 
 def _small(soft_walk, w=16, h=12, ns=60):
@@ -106,7 +110,10 @@ def test_pose_gradient_matches_fp64_central_differences(soft_walk):
     # the FP64 adjoint, to tell FP32 rounding from finite-difference error
     _, lt64 = march.forward64(g64, cams, with_logt=True)
     gp64, _ = march.backward64(g64, cams, lt64, c)
-    eps = 1e-4
+    # eps = 1e-5: the differences' truncation comes from samples that cross
+    # a knot of the C1 spline within +-eps, so it shrinks with eps, while
+    # FP64 round-off of the sums stays ~1e-9 relative
+    eps = 1e-5
     fd = torch.zeros_like(gp)
     for k in range(6):
         xi = torch.zeros((16, 6), dtype=torch.float64, device="cuda")
@@ -116,8 +123,17 @@ def test_pose_gradient_matches_fp64_central_differences(soft_walk):
         fd[:, k] = (lp - lm) / (2 * eps)
     err = (gp - fd).norm(dim=1) / fd.norm(dim=1)
     err64 = (gp64 - fd).norm(dim=1) / fd.norm(dim=1)
-    print("pose gradient rel err per camera: FP32 adjoint", err.cpu().numpy(),
-          "FP64 adjoint", err64.cpu().numpy(), flush=True)
+    err32 = (gp - gp64).norm(dim=1) / gp64.norm(dim=1)
+    print("pose gradient rel err per camera: FP32 adjoint vs FD",
+          err.cpu().numpy(), "FP64 adjoint vs FD", err64.cpu().numpy(),
+          "FP32 vs FP64 adjoint", err32.cpu().numpy(), flush=True)
+    # the adjoint's mathematics: the FP64 adjoint against FP64 central
+    # differences
+    assert float(err64.max()) <= 2e-5
+    # the FP32 product against the same adjoint in FP64: FP32 rounding only
+    # (measured up to 1.7e-5 on these 16 cameras)
+    assert float(err32.max()) <= 5e-5
+    # and end to end
     assert float(err.max()) <= 1e-4
 
 
