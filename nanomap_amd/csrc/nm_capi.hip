@@ -83,6 +83,9 @@ void set_error(const std::string& msg) { g_error = msg; }
 
 }  // namespace nm
 
+// Time slices per ray of the voxel filter's accumulate pass.
+constexpr int kFilterSlices = 4;
+
 namespace {
 
 int fail(int code, const std::string& msg) {
@@ -618,6 +621,10 @@ int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.ray_off = 0;
   p.item_off = 0;
   p.nitems = nm::job_items(p, p.layout);
+  // Voxel-filter rays are few (one per occupied voxel) and long: their
+  // accumulate pass runs each walk in kFilterSlices time slices, for that
+  // many times the work items in flight
+  p.slices = p.use_rays ? kFilterSlices : 1;
 
   nm::Pipeline& pl = ctx->pipe;
   pl = nm::Pipeline{};
@@ -626,6 +633,7 @@ int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   pl.cube_vol = (int64_t)cube.side * cube.side * cube.side;
   pl.n_rays = n_points;
   pl.nitems = p.nitems;
+  pl.accum_items = p.nitems * p.slices;
   pl.tile_cap = ctx->tile_cap;
   pl.sort_bits = nm::sort_bits_for((int64_t)ctx->tile_cap * 512);
   pl.heavy_hits = nm::kFrameHeavyHits;

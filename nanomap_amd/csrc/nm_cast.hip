@@ -284,6 +284,32 @@ __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
   return a;
 }
 
+// Moves one axis of a voxel-DDA state to time `lo`: every crossing at or
+// below lo is applied (the voxel moves one step per crossing) and the next
+// crossing becomes the first one above lo.  The crossing times are the
+// axis's own rounded sequence c(k+1) = c(k) + d, as the voxel loop produces
+// them, eight at a time where a binade allows (axis_leaf); the round-down
+// adds are emulated here, outside the loop's hardware-mode region.
+template <int M>
+__device__ __forceinline__ void advance_axis(double& next, double d, int sgn,
+                                             int& v, double lo) {
+  if (sgn == 0 || !(next <= lo)) return;  // no crossing at or below lo
+  AxisRun a = axis_start<M, false>(next, d, sgn, v, lo);
+  while (a.over == 1.7976931348623157e308) axis_leaf<M, false>(a, lo);
+  next = a.over;
+  v += sgn * a.n;
+}
+
+// The state the voxel loop has once it has taken every step at or below
+// time lo: the three axes' crossing sequences are independent, so each is
+// advanced on its own.
+template <int M>
+__device__ __forceinline__ void dda_advance(Dda& s, double lo) {
+  advance_axis<M>(s.nx, s.dx, s.sx, s.vx, lo);
+  advance_axis<M>(s.ny, s.dy, s.sy, s.vy, lo);
+  advance_axis<M>(s.nz, s.dz, s.sz, s.vz, lo);
+}
+
 // Largest value over the workgroup (256 threads).
 __device__ __forceinline__ uint32_t block_max(uint32_t v) {
   __shared__ uint32_t s_max[4];
@@ -673,15 +699,27 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     __syncthreads();  // the table is clear
     const Job J = jobs[job_of_item(q, item)];
     const int32_t* __restrict__ ls = leaf_slot + J.cube_off;
-    const int i = point_of(J, item - J.item_off, threadIdx.x);
+    // a sliced job (single-job launches) has nsl = `slices` items per ray
+    // block:
+    // slice k of a ray takes the crossings in (T_k, T_k+1], T_k = L k / nsl
+    // (T_0 below every crossing, T_nsl = L), so the slices of a ray count
+    // exactly the misses of its whole walk
+    const int nsl = J.slices > 1 ? J.slices : 1;
+    const int local = item - J.item_off;
+    const int blk = local / nsl, slice = local - blk * nsl;
+    const int i = point_of(J, blk, threadIdx.x);
     const uint32_t gi = (uint32_t)(J.ray_off + i);
     RayDir r;
     if (i >= 0 && ray_of<M>(J, i, r)) {
       Dda s;
       dda_init<M>(J.eye, r, s);
+      double L = r.L;
+      if (nsl > 1) {
+        if (slice > 0) dda_advance<M>(s, r.L * slice / nsl);
+        if (slice < nsl - 1) L = r.L * (slice + 1) / nsl;
+      }
       int px = s.vx, py = s.vy, pz = s.vz;
       int entry = slot_of(J.cube, ls, px, py, pz);
-      double L = r.L;
       if (M == kDown) fp64_round_down_begin(s, L);
       while (dda_step<M, 2, M == kDown>(s, L)) {
         // miss on (px, py, pz)
@@ -1455,15 +1493,16 @@ ItemQueue cast_queue(const Pipeline& p, const Workspace& b, int which,
   q.item_job = p.njobs > 1 ? b.item_job : nullptr;
   q.order = nullptr;
   q.head = nullptr;
-  q.nitems = p.nitems;
-  *grid = p.nitems;
+  const int n = which == 1 && p.accum_items > 0 ? p.accum_items : p.nitems;
+  q.nitems = n;
+  *grid = n;
   if (p.tuning.queue != 0) {
     q.head = &b.counters->queue[which];
     if (p.tuning.queue >= 2) q.order = b.item_order;
     if (p.tuning.queue == 3 && which == 1 && p.accum_order)
       q.order = p.accum_order;
     const int g = p.cu_count * (p.tuning.grid_mult > 0 ? p.tuning.grid_mult : 1);
-    *grid = p.nitems < g ? p.nitems : g;
+    *grid = n < g ? n : g;
   }
   return q;
 }

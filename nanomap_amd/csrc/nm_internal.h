@@ -139,6 +139,9 @@ struct Job {
   int layout;                  // Layout of the work items (tiled only)
   int map;                     // index into the map table
   int depth;                   // 1: `cloud` is a depth image
+  int slices;                  // accumulate work per ray: > 1 cuts each
+                               // walk into that many time slices (one work
+                               // item each); single-job launches only
   // placement in the shared workspace
   int64_t cube_off;  // first entry of the job's cube table (kScanChunk-aligned)
   int ray_off;       // global index of the job's first point
@@ -263,6 +266,8 @@ struct Pipeline {
   int64_t cube_vol;   // workspace cube entries in use
   int64_t n_rays;     // points of all jobs
   int nitems;         // cast work items of all jobs
+  int accum_items = 0;  // work items of the accumulate pass (0: nitems; a
+                        // sliced job has nitems * slices)
   int tile_cap;
   int sort_bits;
   uint32_t heavy_hits = kFrameHeavyHits;  // see kFrameHeavyHits
