@@ -82,11 +82,11 @@ __global__ __launch_bounds__(256) void k_filter_keys(
       atomicAdd(&cnt->errors, 1);
     }
   }
+  // the sort's first pass drops the kInvalid keys and counts the rest
+  // into n_filter_points (a per-wavefront atomic on that one counter
+  // serialised at the memory side: 48 us for a 640x480 frame)
   keys[i] = key;
   vals[i] = (uint32_t)i;
-  const uint64_t m = __ballot(key != kInvalid);
-  if ((threadIdx.x & 63) == 0 && m)
-    atomicAdd(&cnt->n_filter_points, (int)__popcll(m));
 }
 
 // precision: 0 FP32 centroid, 1 FP16 centroid, 2 voxel centre (int
@@ -108,28 +108,67 @@ __global__ __launch_bounds__(256) void k_filter_bins(
       const int bx = (int)((key >> (2 * fk.bits)) & m) - fk.offset;
       const int by = (int)((key >> fk.bits) & m) - fk.offset;
       const int bz = (int)(key & m) - fk.offset;
+      // the bin is keys[e, end): a galloping search finds its end in
+      // O(log n) loads instead of one dependent load per point
+      int end;
+      {
+        int lo = e + 1, hi = nv, step = 1;
+        for (;;) {
+          const int probe = e + step;
+          if (probe >= nv) break;
+          if (keys[probe] != key) {
+            hi = probe;
+            break;
+          }
+          lo = probe + 1;
+          step <<= 1;
+        }
+        while (lo < hi) {  // first index in [lo, hi) outside the bin
+          const int mid = (lo + hi) >> 1;
+          if (keys[mid] == key)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        end = lo;
+      }
       float pt[3];
       bool emit = true;
       if (precision <= 1) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         _Float16 hx = 0, hy = 0, hz = 0, hn = 0;
-        uint32_t n = 0;
-        for (int q = e; q < nv && keys[q] == key; ++q) {
-          bool valid;
-          int b[3];
-          float f[3];
-          point_bin(p, fb, origin, (int)vals[q], valid, b, f);
-          if (precision == 0) {
-            sx += f[0];
-            sy += f[1];
-            sz += f[2];
-          } else {  // each term rounded to FP16, each sum in FP16
-            hx = hx + (_Float16)f[0];
-            hy = hy + (_Float16)f[1];
-            hz = hz + (_Float16)f[2];
-            hn = hn + (_Float16)1.0f;
+        const uint32_t n = (uint32_t)(end - e);
+        // points are read eight at a time so their loads overlap; the sums
+        // still take them one by one in ascending pixel order
+        for (int q0 = e; q0 < end; q0 += 8) {
+          float fx[8], fy[8], fz[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (q0 + u < end) {
+              bool valid;
+              int b[3];
+              float f[3];
+              point_bin(p, fb, origin, (int)vals[q0 + u], valid, b, f);
+              fx[u] = f[0];
+              fy[u] = f[1];
+              fz[u] = f[2];
+            }
           }
-          ++n;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (q0 + u < end) {
+              if (precision == 0) {
+                sx += fx[u];
+                sy += fy[u];
+                sz += fz[u];
+              } else {  // each term rounded to FP16, each sum in FP16
+                hx = hx + (_Float16)fx[u];
+                hy = hy + (_Float16)fy[u];
+                hz = hz + (_Float16)fz[u];
+                hn = hn + (_Float16)1.0f;
+              }
+            }
+          }
         }
         if (precision == 1) {
           sx = (float)hx;
@@ -141,8 +180,7 @@ __global__ __launch_bounds__(256) void k_filter_bins(
         pt[1] = sy / c + (float)by;
         pt[2] = sz / c + (float)bz;
       } else {
-        uint32_t n = 0;
-        for (int q = e; q < nv && keys[q] == key; ++q) ++n;
+        const uint32_t n = (uint32_t)(end - e);
         emit = precision == 2 || (n & 255u) != 0u;
         pt[0] = (float)bx + 0.5f;
         pt[1] = (float)by + 0.5f;
@@ -254,7 +292,7 @@ hipError_t launch_voxel_filter(const CastParams& p, const float origin[3],
   uint32_t* vals[2] = {b.vals[0], b.vals[1]};
   int out = 0;
   hipError_t e = radix_sort_pairs(keys, vals, p.n, 3 * fk.bits, b.radix_hist,
-                                  &out, s);
+                                  &out, s, &b.counters->n_filter_points);
   if (e != hipSuccess) return e;
   k_filter_bins<<<min(grid, 4096), 256, 0, s>>>(
       p, fb, origin[0], origin[1], origin[2], fk, precision, b.keys[out],
