@@ -153,14 +153,14 @@ __global__ __launch_bounds__(256) void k_zero_i32(int* __restrict__ p) {
   if (threadIdx.x == 0) *p = 0;
 }
 
-__global__ __launch_bounds__(256) void k_sim_cast(
-    SimGridView g, const SimDesc* __restrict__ descs,
-    const float* __restrict__ tables, int n_desc, int total_tiles,
+// One 8x8 pixel tile per wavefront; returns the tile's return count (the
+// same value in every lane).
+__device__ __forceinline__ int sim_tile(
+    const SimGridView& g, const SimDesc* __restrict__ descs,
+    const float* __restrict__ tables, int n_desc, int tile,
     int32_t* __restrict__ out_ijk, float* __restrict__ out_dist,
-    float* __restrict__ out_xyz, int* __restrict__ returns) {
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    float* __restrict__ out_xyz) {
   const int lane = threadIdx.x & 63;
-  if (tile >= total_tiles) return;  // whole wavefronts
   // sensor of this tile (few sensors: linear scan)
   int s = 0;
   while (s + 1 < n_desc && descs[s + 1].tile_off <= tile) ++s;
@@ -328,10 +328,8 @@ __global__ __launch_bounds__(256) void k_sim_cast(
     }
   }
   const bool ret = in_image && hit && thit >= min_t && thit <= max_t;
-  // one atomic per wavefront for the return count
-  const uint64_t rb = __ballot(ret);
-  if (lane == 0 && rb) atomicAdd(returns, (int)__popcll(rb));
-  if (!in_image) return;
+  const int count = (int)__popcll(__ballot(ret));
+  if (!in_image) return count;
   if (ret) {
     const float dist = thit * d.res;
     if (out_ijk) {
@@ -357,6 +355,32 @@ __global__ __launch_bounds__(256) void k_sim_cast(
       out_xyz[3 * pix + 1] = __builtin_nanf("");
       out_xyz[3 * pix + 2] = __builtin_nanf("");
     }
+  }
+  return count;
+}
+
+// Tiles in a grid-stride loop over kSimBlocks workgroups at most; the
+// return count is summed per wavefront, then per workgroup, and added with
+// one atomic per workgroup (one per wavefront and tile serialised on the
+// counter at the memory side).
+constexpr int kSimBlocks = 16384;
+
+__global__ __launch_bounds__(256) void k_sim_cast(
+    SimGridView g, const SimDesc* __restrict__ descs,
+    const float* __restrict__ tables, int n_desc, int total_tiles,
+    int32_t* __restrict__ out_ijk, float* __restrict__ out_dist,
+    float* __restrict__ out_xyz, int* __restrict__ returns) {
+  __shared__ int s_count[4];
+  int count = 0;
+  for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < total_tiles;
+       tile += gridDim.x * 4)
+    count += sim_tile(g, descs, tables, n_desc, tile, out_ijk, out_dist,
+                      out_xyz);
+  if ((threadIdx.x & 63) == 0) s_count[threadIdx.x >> 6] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int total = s_count[0] + s_count[1] + s_count[2] + s_count[3];
+    if (total) atomicAdd(returns, total);
   }
 }
 
@@ -496,8 +520,8 @@ hipError_t sim_launch(const nm_simgrid* g, const SimLaunch& L,
       reinterpret_cast<const float*>(scratch + kSimImageOffset + L.tables_off);
   k_zero_i32<<<1, 64, 0, s>>>(ret);
   if (L.tiles > 0)
-    k_sim_cast<<<(L.tiles + 3) / 4, 256, 0, s>>>(v, descs, tables, L.n_desc,
-                                                 L.tiles, ijk, dist, xyz, ret);
+    k_sim_cast<<<std::min((L.tiles + 3) / 4, kSimBlocks), 256, 0, s>>>(
+        v, descs, tables, L.n_desc, L.tiles, ijk, dist, xyz, ret);
   return hipGetLastError();
 }
 
