@@ -3,16 +3,18 @@
 // Mirrors include/nanomap/map/OccupancyMap.h:35-61: the constructor takes
 // the voxel size and the hit/miss probability thresholds and derives the
 // log-odds thresholds and the clamping values logit(0.97) / logit(0.12).
-// The grid lives in device memory (nm_map): getValue()/query() read voxels
-// through device point queries in place of the reference's OpenVDB
-// accessor, leaves() copies the whole grid to the host, save() and the
-// snapshot constructor store and restore it.
+// The grid lives in device memory (nm_map): getValue()/query() and the
+// occAccessor() read voxels through device point queries in place of the
+// reference's OpenVDB accessor, leaves() and occupiedGrid() copy the whole
+// grid to the host, save() and the snapshot constructor store and restore
+// it.
 #ifndef NANOMAP_AMD_HOST_MAP_H
 #define NANOMAP_AMD_HOST_MAP_H
 
 #include <cmath>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -21,6 +23,54 @@
 
 namespace nanomap {
 namespace map {
+
+// Voxel reads with the method names of the OpenVDB accessor that the
+// reference's occAccessor() returns (OccupancyMap.h:60): each call is one
+// device point query on the map of record.  Coordinates are index space.
+class Accessor {
+ public:
+  explicit Accessor(nm_map* m) : _m(m) {}
+  // log-odds of voxel (i, j, k); 0 (the background) where its leaf is absent
+  float getValue(int i, int j, int k) const {
+    float v;
+    bool a;
+    read(i, j, k, v, a);
+    return v;
+  }
+  bool isValueOn(int i, int j, int k) const {
+    float v;
+    bool a;
+    read(i, j, k, v, a);
+    return a;
+  }
+  // value into `value`, returns the active state
+  bool probeValue(int i, int j, int k, float& value) const {
+    bool a;
+    read(i, j, k, value, a);
+    return a;
+  }
+
+ private:
+  void read(int i, int j, int k, float& v, bool& a) const {
+    const int32_t ijk[3] = {i, j, k};
+    uint8_t act = 0;
+    check(nm_map_query(_m, ijk, 1, &v, &act, 0, nullptr));
+    a = act != 0;
+  }
+  nm_map* _m;
+};
+
+// Host snapshot of every leaf, the stand-in for the OpenVDB grid that the
+// reference's occupiedGrid() returns (OccupancyMap.h:54): origins (index
+// space, multiples of 8), 512 log-odds per leaf (voxel z + 8 y + 64 x) and
+// 16 active-mask words per leaf.
+struct LeafGrid {
+  float voxelSize = 0.0f;
+  std::vector<int32_t> origins;
+  std::vector<float> values;
+  std::vector<uint32_t> masks;
+  int64_t leafCount() const { return (int64_t)(origins.size() / 3); }
+};
 
 class Map {
  public:
@@ -98,6 +148,18 @@ class Map {
   void query(const int32_t* ijk, int64_t n, float* values,
              uint8_t* active) const {
     check(nm_map_query(_map, ijk, n, values, active, 0, nullptr));
+  }
+
+  // OccupancyMap.h:60: a voxel accessor (device point queries)
+  std::shared_ptr<Accessor> occAccessor() const {
+    return std::make_shared<Accessor>(_map);
+  }
+  // OccupancyMap.h:54: the grid, as a host copy of every leaf
+  std::shared_ptr<LeafGrid> occupiedGrid() const {
+    auto g = std::make_shared<LeafGrid>();
+    g->voxelSize = _gridRes;
+    leaves(g->origins, g->values, g->masks);
+    return g;
   }
 
   nm_map* handle() { return _map; }

@@ -22,6 +22,7 @@
 #include <sstream>
 #include <vector>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -94,6 +95,29 @@ static int map(int argc, char** argv) {
   std::vector<uint32_t> mask;
   m->leaves(o, val, mask);
   const int64_t n = (int64_t)(o.size() / 3);
+  // the reference's read paths: occupiedGrid() holds the same leaves, and
+  // occAccessor() reads the same values and states voxel by voxel
+  auto grid = m->occupiedGrid();
+  if (grid->leafCount() != n || grid->values != val || grid->masks != mask) {
+    std::fprintf(stderr, "occupiedGrid differs from leaves()\n");
+    return 5;
+  }
+  auto acc = m->occAccessor();
+  for (int64_t l = 0; l < n; l += std::max<int64_t>(1, n / 16)) {
+    for (int v = 0; v < 512; v += 37) {
+      const int i = o[3 * l] + (v >> 6), j = o[3 * l + 1] + ((v >> 3) & 7),
+                k = o[3 * l + 2] + (v & 7);
+      float pv = 0.0f;
+      const bool on = acc->probeValue(i, j, k, pv);
+      const bool want_on = (mask[16 * l + (v >> 5)] >> (v & 31)) & 1u;
+      if (pv != val[512 * l + v] || acc->getValue(i, j, k) != pv ||
+          on != want_on || acc->isValueOn(i, j, k) != on) {
+        std::fprintf(stderr, "occAccessor differs at leaf %lld voxel %d\n",
+                     (long long)l, v);
+        return 5;
+      }
+    }
+  }
   std::ofstream out(argv[6], std::ios::binary);
   out.write(reinterpret_cast<const char*>(&n), 8);
   out.write(reinterpret_cast<const char*>(o.data()), o.size() * 4);

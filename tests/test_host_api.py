@@ -160,6 +160,8 @@ def _read_map(path):
                           (0, 0, 0, 16), (1, 0, 1, 8)])
 def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
                                        precision, update_type, capacity):
+    # The driver also checks map::Map's occupiedGrid() against leaves() and
+    # occAccessor() reads against the leaf values and masks (exit code 5).
     # FilterType 1 with PrecisionType 0-3 selects the four voxel filters,
     # which drop points outside the frustum's sensor-frame box.  UpdateType
     # 1 (the reference's per-leaf BlockWorker fold, BlockWorker.cpp:95-176)
