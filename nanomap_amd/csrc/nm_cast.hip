@@ -1202,62 +1202,90 @@ __global__ void k_runs_mark(const Job* __restrict__ jobs, int njobs,
     runs_key[j] = runs_key_of(jobs[j].miss, M);
 }
 
+// Pixel-order temp value of a voxel with k misses and the j sorted hits
+// gap[s0, s0 + j) (gap g: the misses before hit g): the gap runs and hits
+// folded one by one, then the misses after the last hit.
+template <int M>
+__device__ __forceinline__ float light_temp(const uint32_t* __restrict__ gap,
+                                            uint32_t s0, uint32_t j,
+                                            uint32_t k, float hit,
+                                            float miss) {
+  // gaps are loaded 8 at a time so the loads overlap
+  uint32_t used = 0;
+  float acc = 0.0f;
+  for (uint32_t g = 0; g < j; g += 8) {
+    uint32_t gm[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (g + u < j) {
+        if (gm[u] != 0u) {
+          acc = fold_small<M>(acc, miss, gm[u]);
+          used += gm[u];
+        }
+        acc = add<M, true>(acc, hit);
+      }
+    }
+  }
+  // (used <= k: every gap counts a miss that k counts too)
+  return fold<M, true>(acc, miss, k >= used ? k - used : 0u);
+}
+
+// Temp values of every voxel with at most heavy_hits hits, in one launch
+// of two parts that run side by side:
+//  - blocks [0, lblocks): the voxels with 1..heavy_hits hits, one lane per
+//    voxel, found where its run starts in the sorted hit keys, so the lanes
+//    of a wavefront are all hit voxels and none idles through another's
+//    loop (a run longer than heavy_hits is k_heavy's, which may already
+//    have rewritten the word: j then exceeds heavy_hits as well);
+//  - the other blocks: the voxels without hits, one lane per voxel of every
+//    tile, the miss-only sum fold(0, miss, k) from the job's table (a voxel
+//    holding hits has a nonzero hit word).
 template <int M>
 __global__ __launch_bounds__(256) void k_temp(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     uint32_t* __restrict__ tiles, const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ gap, const FrameCounters* __restrict__ cnt,
-    int tile_cap, uint32_t heavy_hits, const float* __restrict__ miss_runs) {
+    const uint32_t* __restrict__ keys, uint32_t heavy_hits, int tile_cap,
+    const float* __restrict__ miss_runs, int lblocks) {
+  if ((int)blockIdx.x < lblocks) {
+    const int nh = cnt->n_hit_keys;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < nh; e += lblocks * 256) {
+      const uint32_t key = keys[e];
+      if (key == kInvalid || (e > 0 && keys[e - 1] == key)) continue;
+      const size_t cell = key;
+      const uint32_t s0 = hit_start[cell];
+      const uint32_t end = tiles[2 * cell + 1];
+      const uint32_t j = end - s0;
+      if (j == 0u || j > heavy_hits) continue;
+      const Job& J = jobs[slot_job[cell >> 9]];
+      const uint32_t k = tiles[2 * cell];
+      const float tv = light_temp<M>(gap, s0, j, k, J.hit, J.miss);
+      tiles[2 * cell] = bits_of(tv);
+      tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+    }
+    return;
+  }
   const int lane = threadIdx.x & 63;
   const int64_t nvox = (int64_t)min(cnt->n_leaves, tile_cap) * 512;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int64_t stride = (int64_t)(gridDim.x - lblocks) * blockDim.x;
+  for (int64_t base = (int64_t)(blockIdx.x - lblocks) * blockDim.x +
+                      (threadIdx.x & ~63);
        base < nvox; base += stride) {
     // the wavefront's 64 voxels lie in one tile
     const int slot = (int)(base >> 9);
     const int job = __builtin_amdgcn_readfirstlane(slot_job[slot]);
-    const Job& J = jobs[job];
-    const float hit = J.hit, miss = J.miss;
+    const float miss = jobs[job].miss;
     const size_t cell = (size_t)base + lane;
     const uint32_t k = tiles[2 * cell];
     const uint32_t end = tiles[2 * cell + 1];
-    const uint32_t s0 = end ? hit_start[cell] : 0u;
-    const uint32_t j = end - s0;
-    const bool touched = (k | j) != 0;
-    // voxels with more than heavy_hits hits are k_heavy's.  k_heavy may
-    // run alongside: a hit word it has already
-    // rewritten carries kTouchedBit (a run end is below 2^30 points), and
-    // such a voxel is skipped too.
-    if (!touched || j > heavy_hits || end >= kTouchedBit) continue;
-    float tv;
-    if (j == 0) {
-      tv = k < (uint32_t)kMissRuns ? miss_runs[(size_t)job * kMissRuns + k]
-                                   : fold<M, true>(0.0f, miss, k);
-    } else {
-      // gaps are loaded 8 at a time so the loads overlap
-      uint32_t used = 0;
-      float acc = 0.0f;
-      for (uint32_t g = 0; g < j; g += 8) {
-        uint32_t gm[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          gm[u] = g + u < j ? gap[s0 + g + u] : 0u;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (g + u < j) {
-            if (gm[u] != 0u) {
-              acc = fold_small<M>(acc, miss, gm[u]);
-              used += gm[u];
-            }
-            acc = add<M, true>(acc, hit);
-          }
-        }
-      }
-      // (used <= k: every gap counts a miss that k counts too)
-      tv = fold<M, true>(acc, miss, k >= used ? k - used : 0u);
-    }
+    if (k == 0u || end != 0u) continue;  // untouched, or holding hits
+    const float tv = k < (uint32_t)kMissRuns
+                         ? miss_runs[(size_t)job * kMissRuns + k]
+                         : fold<M, true>(0.0f, miss, k);
     tiles[2 * cell] = bits_of(tv);
-    tiles[2 * cell + 1] = j | kTouchedBit | (k != 0 ? kMixedBit : 0u);
+    tiles[2 * cell + 1] = kTouchedBit | kMixedBit;
   }
 }
 
@@ -1699,15 +1727,15 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   } else {
     map_lookup_and_runs(p, b, map_table, single, s);
   }
+  const int lgrid = p.cu_count * 4;
   if (down)
-    k_temp<kDown><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
-                                        b.hit_start, b.gap, b.counters,
-                                        p.tile_cap, p.heavy_hits, b.miss_runs);
+    k_temp<kDown><<<lgrid + tgrid, 256, 0, s>>>(
+        b.jobs, b.slot_job, b.tiles, b.hit_start, b.gap, b.counters,
+        b.keys[sorted_buf], p.heavy_hits, p.tile_cap, b.miss_runs, lgrid);
   else
-    k_temp<kNearest><<<tgrid, 256, 0, s>>>(b.jobs, b.slot_job, b.tiles,
-                                           b.hit_start, b.gap, b.counters,
-                                           p.tile_cap, p.heavy_hits,
-                                           b.miss_runs);
+    k_temp<kNearest><<<lgrid + tgrid, 256, 0, s>>>(
+        b.jobs, b.slot_job, b.tiles, b.hit_start, b.gap, b.counters,
+        b.keys[sorted_buf], p.heavy_hits, p.tile_cap, b.miss_runs, lgrid);
   if (p.side) {
     const hipError_t e = hipStreamWaitEvent(s, p.join, 0);
     if (e != hipSuccess) return e;
