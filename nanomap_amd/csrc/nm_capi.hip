@@ -83,8 +83,11 @@ void set_error(const std::string& msg) { g_error = msg; }
 
 }  // namespace nm
 
-// Time slices per ray of the voxel filter's accumulate pass.
-constexpr int kFilterSlices = 4;
+// Time slices per ray block of the accumulate pass, at most: a frame's
+// work items all start at once, so the pass lasts as long as its longest
+// rays, and the blocks that hold them are cut into slices (one per 50
+// visits of their longest ray, see k_accumulate).
+constexpr int kAccumSlices = 6;
 
 namespace {
 
@@ -621,10 +624,7 @@ int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.ray_off = 0;
   p.item_off = 0;
   p.nitems = nm::job_items(p, p.layout);
-  // Voxel-filter rays are few (one per occupied voxel) and long: their
-  // accumulate pass runs each walk in kFilterSlices time slices, for that
-  // many times the work items in flight
-  p.slices = p.use_rays ? kFilterSlices : 1;
+  p.slices = kAccumSlices;
 
   nm::Pipeline& pl = ctx->pipe;
   pl = nm::Pipeline{};

@@ -284,6 +284,10 @@ __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
   return a;
 }
 
+// Voxel visits per time slice of a sliced accumulate (a ray block whose
+// longest ray has v visits is cut into min(slices, v / kSliceVisits)).
+constexpr uint32_t kSliceVisits = 40;
+
 // Moves one axis of a voxel-DDA state to time `lo`: every crossing at or
 // below lo is applied (the voxel moves one step per crossing) and the next
 // crossing becomes the first one above lo.  The crossing times are the
@@ -333,7 +337,8 @@ template <int M>
 __global__ __launch_bounds__(kCastThreads) void k_discover(
     const Job* __restrict__ jobs, ItemQueue q, int32_t* __restrict__ leaf_slot,
     uint32_t* __restrict__ ray_hit, uint64_t* __restrict__ part,
-    uint32_t* __restrict__ cost_keys, uint32_t* __restrict__ cost_vals) {
+    uint32_t* __restrict__ cost_keys, uint32_t* __restrict__ cost_vals,
+    uint32_t* __restrict__ item_longest) {
   int round = 0;
   for (;;) {
     const int item = take_item(q, round);
@@ -400,11 +405,14 @@ __global__ __launch_bounds__(kCastThreads) void k_discover(
     const uint64_t v[4] = {visits, (uint64_t)valid,
                            (uint64_t)(hit_cell != kInvalid), (uint64_t)errors};
     block_partials(part + (size_t)item * 4, v, 4);
-    if (cost_keys) {
+    if (cost_keys || item_longest) {
       const uint32_t longest = block_max(visits);
       if (threadIdx.x == 0) {
-        cost_keys[item] = cost_key(longest);
-        cost_vals[item] = (uint32_t)item;
+        if (cost_keys) {
+          cost_keys[item] = cost_key(longest);
+          cost_vals[item] = (uint32_t)item;
+        }
+        if (item_longest) item_longest[item] = longest;
       }
     }
   }
@@ -683,7 +691,8 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const uint32_t* __restrict__ hit_mask,
     const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
-    FrameCounters* __restrict__ cnt) {
+    FrameCounters* __restrict__ cnt,
+    const uint32_t* __restrict__ item_longest) {
   constexpr int kSlots = 1 << S;
   __shared__ uint32_t s_key[kSlots];
   __shared__ uint32_t s_cnt[kSlots / 2];
@@ -704,9 +713,16 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     // slice k of a ray takes the crossings in (T_k, T_k+1], T_k = L k / nsl
     // (T_0 below every crossing, T_nsl = L), so the slices of a ray count
     // exactly the misses of its whole walk
-    const int nsl = J.slices > 1 ? J.slices : 1;
+    const int nmax = J.slices > 1 ? J.slices : 1;
     const int local = item - J.item_off;
-    const int blk = local / nsl, slice = local - blk * nsl;
+    const int blk = local / nmax, slice = local - blk * nmax;
+    // as many slices as the block's longest ray needs (the others exit)
+    int nsl = 1;
+    if (nmax > 1) {
+      const uint32_t longest = item_longest[J.item_off + blk];
+      nsl = (int)min((uint32_t)nmax, max(1u, longest / kSliceVisits));
+    }
+    if (slice >= nsl) continue;
     const int i = point_of(J, blk, threadIdx.x);
     const uint32_t gi = (uint32_t)(J.ray_off + i);
     RayDir r;
@@ -1561,12 +1577,13 @@ hipError_t launch_discover(const Pipeline& p, const Workspace& b,
   const ItemQueue q = cast_queue(p, b, 0, &grid);
   uint32_t* ck = p.tuning.queue == 3 ? b.cost_keys[0] : nullptr;
   uint32_t* cv = p.tuning.queue == 3 ? b.cost_vals[0] : nullptr;
+  uint32_t* il = p.accum_items > p.nitems ? b.item_longest : nullptr;
   if (p.rounding == kDown)
     k_discover<kDown><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv);
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv, il);
   else
     k_discover<kNearest><<<grid, kCastThreads, 0, s>>>(
-        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv);
+        b.jobs, q, b.leaf_slot, b.ray_hit, b.part_discover, ck, cv, il);
   return hipGetLastError();
 }
 
@@ -1626,7 +1643,7 @@ void accumulate_as(const Pipeline& p, const Workspace& b, int sorted_buf,
                    const ItemQueue& q, int grid, hipStream_t s) {
   k_accumulate<M, S><<<grid, kCastThreads, 0, s>>>(
       b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
-      b.vals[sorted_buf], b.gap, b.counters);
+      b.vals[sorted_buf], b.gap, b.counters, b.item_longest);
 }
 
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,

@@ -139,9 +139,11 @@ struct Job {
   int layout;                  // Layout of the work items (tiled only)
   int map;                     // index into the map table
   int depth;                   // 1: `cloud` is a depth image
-  int slices;                  // accumulate work per ray: > 1 cuts each
-                               // walk into that many time slices (one work
-                               // item each); single-job launches only
+  int slices;                  // accumulate work per ray block: up to
+                               // this many time slices (one work item each,
+                               // as many as the block's longest ray needs:
+                               // one per kSliceVisits visits); single-job
+                               // launches only
   // placement in the shared workspace
   int64_t cube_off;  // first entry of the job's cube table (kScanChunk-aligned)
   int ray_off;       // global index of the job's first point
@@ -213,6 +215,8 @@ struct Workspace {
   uint64_t* part_discover;  // [item_cap * 4] per-item counter sums
   int32_t* item_job;    // [item_cap]  job of each work item
   int32_t* item_order;  // [item_cap]  queue order of the work items
+  uint32_t* item_longest;  // [item_cap]  visits of each item's longest ray
+                           //   (sliced accumulate)
   uint32_t* cost_keys[2];  // [item_cap]  item cost keys (queue 3) and the
   uint32_t* cost_vals[2];  //             item indices sorted by them
   unsigned char* cloud; // staging for host clouds
