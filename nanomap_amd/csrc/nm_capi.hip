@@ -34,6 +34,7 @@ struct nm_ctx {
   int64_t cube_cap = 0;
   int tile_cap = 0;
   int cu_count = 256;
+  int resident = 1536;  // accumulate workgroups resident at once
   size_t staging_bytes = 0;
   nm::Workspace b{};
   nm::FrameCounters* host_counters = nullptr;
@@ -83,11 +84,15 @@ void set_error(const std::string& msg) { g_error = msg; }
 
 }  // namespace nm
 
-// Time slices per ray block of the accumulate pass, at most: a frame's
-// work items all start at once, so the pass lasts as long as its longest
-// rays, and the blocks that hold them are cut into slices (one per 50
-// visits of their longest ray, see k_accumulate).
+// Sliced accumulate: a frame's work items all start at once when they fit
+// on the GPU, so the pass lasts as long as its longest rays.  The blocks
+// that hold them are cut into up to kAccumSlices time slices, run as extra
+// work items: as many as kSliceOvercommit / 4 times the resident
+// accumulate workgroups leave beside the frame's blocks (none when the
+// blocks alone fill it), shared out by the slice plan.
 constexpr int kAccumSlices = 6;
+constexpr int kSliceOvercommit = 8;
+static_assert(kAccumSlices <= nm::kMaxSlices, "slice index takes 3 bits");
 
 namespace {
 
@@ -274,6 +279,7 @@ int nm_ctx_create(const nm_ctx_config* cfg, nm_ctx** out) {
   if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess &&
       prop.multiProcessorCount > 0)
     c->cu_count = prop.multiProcessorCount;
+  c->resident = nm::accumulate_resident(c->cu_count);
   hipError_t e = nm::workspace_alloc(d, &c->b);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&c->host_counters, sizeof(nm::FrameCounters));
@@ -624,7 +630,6 @@ int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   p.ray_off = 0;
   p.item_off = 0;
   p.nitems = nm::job_items(p, p.layout);
-  p.slices = kAccumSlices;
 
   nm::Pipeline& pl = ctx->pipe;
   pl = nm::Pipeline{};
@@ -633,7 +638,14 @@ int cast_impl(nm_ctx* ctx, const nm_frame* frame, const void* cloud,
   pl.cube_vol = (int64_t)cube.side * cube.side * cube.side;
   pl.n_rays = n_points;
   pl.nitems = p.nitems;
-  pl.accum_items = p.nitems * p.slices;
+  {
+    const int64_t room =
+        (int64_t)ctx->resident * kSliceOvercommit / 4 - p.nitems;
+    const int64_t extra = std::min<int64_t>(
+        std::max<int64_t>(room, 0), (int64_t)p.nitems * (kAccumSlices - 1));
+    pl.accum_items = p.nitems + (int)extra;
+    p.slices = extra > 0 ? kAccumSlices : 1;
+  }
   pl.tile_cap = ctx->tile_cap;
   pl.sort_bits = nm::sort_bits_for((int64_t)ctx->tile_cap * 512);
   pl.heavy_hits = nm::kFrameHeavyHits;

@@ -88,6 +88,20 @@ __device__ __forceinline__ bool ray_of(const Job& p, int i, RayDir& r) {
   return ray_from_point<M>(p.R, x, y, z, p.res, p.max_time, r);
 }
 
+// Inclusive prefix sum over the 64 lanes of a fully active wavefront, in
+// DPP moves: lanes read lane - 1, 2, 4, 8 inside their 16-lane row (0 past
+// the row start), then lane 15 of each row is added to the next row (rows
+// 1 and 3) and lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -284,10 +298,6 @@ __device__ __forceinline__ AxisRun axis_start(double c1, double d, int s,
   return a;
 }
 
-// Voxel visits per time slice of a sliced accumulate (a ray block whose
-// longest ray has v visits is cut into min(slices, v / kSliceVisits)).
-constexpr uint32_t kSliceVisits = 40;
-
 // Moves one axis of a voxel-DDA state to time `lo`: every crossing at or
 // below lo is applied (the voxel moves one step per crossing) and the next
 // crossing becomes the first one above lo.  The crossing times are the
@@ -438,10 +448,93 @@ __global__ __launch_bounds__(256) void k_scan_count(
 // One workgroup: exclusive scan of block_counts; total -> n_leaves.  Also
 // reduces the discover partials of all items (visits, rays, hit rays,
 // errors) into the sequence counters.
+// Sliced accumulate: the visits per time slice V are picked per frame
+// from this ladder -- the first whose extra slice items fit in the extra
+// work items the accumulate launch has -- so a frame whose ray blocks
+// already fill the GPU gets no extra items and is not sliced.  A ray block
+// whose longest ray has v visits is walked in min(nmax, max(1, v / V))
+// time slices.
+constexpr int kSliceLadder = 12;
+__device__ __forceinline__ constexpr uint32_t slice_ladder(int q) {
+  constexpr uint32_t kV[kSliceLadder] = {40,  48,  56,  64,  80,  96,
+                                         128, 160, 200, 256, 320, 400};
+  return kV[q];
+}
+
+__device__ __forceinline__ uint32_t block_slices(uint32_t longest,
+                                                 uint32_t per, int nmax) {
+  return min((uint32_t)nmax, max(1u, longest / per));
+}
+
+// Slice plan, run by one 1024-thread workgroup after the discover pass:
+// picks V and lists the extra work items (every slice but the first of
+// each ray block, as block << 3 | slice, in block order).
+__device__ void plan_slices(const uint32_t* __restrict__ longest, int nblk,
+                            int nmax, int extra,
+                            FrameCounters* __restrict__ cnt,
+                            uint32_t* __restrict__ items) {
+  __shared__ uint32_t s_ex[kSliceLadder][16];
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_pick;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int chunk = (nblk + 1023) / 1024;
+  const int b0 = min(tid * chunk, nblk), b1 = min(b0 + chunk, nblk);
+  uint32_t ex[kSliceLadder];
+#pragma unroll
+  for (int q = 0; q < kSliceLadder; ++q) ex[q] = 0;
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t v = longest[b];
+#pragma unroll
+    for (int q = 0; q < kSliceLadder; ++q)
+      ex[q] += block_slices(v, slice_ladder(q), nmax) - 1u;
+  }
+#pragma unroll
+  for (int q = 0; q < kSliceLadder; ++q) {
+    const uint32_t w = wave_incl_scan32(ex[q]);
+    if (lane == 63) s_ex[q][wave] = w;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t tot = 0;
+    if (lane < kSliceLadder)
+      for (int w = 0; w < 16; ++w) tot += s_ex[lane][w];
+    const uint64_t fits =
+        __ballot(lane < kSliceLadder && tot <= (uint32_t)extra);
+    if (lane == 0) {
+      uint32_t pick = 0;  // no ladder entry fits: no slicing
+      if (fits) {
+        const int q = __builtin_ctzll(fits);
+#pragma unroll
+        for (int k = 0; k < kSliceLadder; ++k)
+          if (k == q) pick = slice_ladder(k);
+      }
+      s_pick = pick;
+      cnt->slice_visits = pick;
+    }
+  }
+  __syncthreads();
+  const uint32_t per = s_pick;
+  if (per == 0u) return;  // slice_extra stays 0
+  uint32_t mine = 0;
+  for (int b = b0; b < b1; ++b)
+    mine += block_slices(longest[b], per, nmax) - 1u;
+  const uint32_t x = wave_incl_scan32(mine);
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t pos = x - mine;
+  for (int w = 0; w < wave; ++w) pos += s_w[w];
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t n = block_slices(longest[b], per, nmax);
+    for (uint32_t sl = 1; sl < n; ++sl) items[pos++] = ((uint32_t)b << 3) | sl;
+  }
+  if (tid == 1023) cnt->slice_extra = (int32_t)(pos);
+}
+
 __global__ __launch_bounds__(1024) void k_scan_blocks(
     int32_t* __restrict__ block_counts, int nblocks,
     const uint64_t* __restrict__ part, int npart,
-    FrameCounters* __restrict__ cnt) {
+    FrameCounters* __restrict__ cnt, const uint32_t* __restrict__ longest,
+    int nmax, int extra, uint32_t* __restrict__ slice_items) {
   __shared__ int wsum[16];
   __shared__ unsigned long long psum[4];
   const int tid = threadIdx.x;
@@ -468,11 +561,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(
   int local = 0;
   for (int b = b0; b < b1; ++b) local += block_counts[b];
   const int lane = tid & 63, wave = tid >> 6;
-  int x = local;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
+  const int x = (int)wave_incl_scan32((uint32_t)local);
   if (lane == 63) wsum[wave] = x;
   __syncthreads();
   if (tid < 16) {
@@ -491,6 +580,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(
     run += v;
   }
   if (tid == 1023) cnt->n_leaves = wsum[15];
+  if (longest) plan_slices(longest, npart, nmax, extra, cnt, slice_items);
 }
 
 __global__ __launch_bounds__(256) void k_scan_assign(
@@ -692,7 +782,8 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     const uint32_t* __restrict__ hit_start,
     const uint32_t* __restrict__ svals, uint32_t* __restrict__ gap,
     FrameCounters* __restrict__ cnt,
-    const uint32_t* __restrict__ item_longest) {
+    const uint32_t* __restrict__ item_longest,
+    const uint32_t* __restrict__ slice_items) {
   constexpr int kSlots = 1 << S;
   __shared__ uint32_t s_key[kSlots];
   __shared__ uint32_t s_cnt[kSlots / 2];
@@ -708,21 +799,28 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
     __syncthreads();  // the table is clear
     const Job J = jobs[job_of_item(q, item)];
     const int32_t* __restrict__ ls = leaf_slot + J.cube_off;
-    // a sliced job (single-job launches) has nsl = `slices` items per ray
-    // block:
-    // slice k of a ray takes the crossings in (T_k, T_k+1], T_k = L k / nsl
-    // (T_0 below every crossing, T_nsl = L), so the slices of a ray count
-    // exactly the misses of its whole walk
+    // a sliced job (single-job launches) walks a ray block in nsl time
+    // slices: slice k of a ray takes the crossings in (T_k, T_k+1],
+    // T_k = L k / nsl (T_0 below every crossing, T_nsl = L), so the slices
+    // of a ray count exactly the misses of its whole walk.  Items
+    // [0, nitems) are the blocks' first slices; the extra items that
+    // k_slice_plan lists are their other slices.
     const int nmax = J.slices > 1 ? J.slices : 1;
-    const int local = item - J.item_off;
-    const int blk = local / nmax, slice = local - blk * nmax;
-    // as many slices as the block's longest ray needs (the others exit)
+    int blk = item - J.item_off, slice = 0;
+    if (blk >= J.nitems) {
+      const int k = blk - J.nitems;
+      if (k >= cnt->slice_extra) continue;
+      const uint32_t e = slice_items[k];
+      blk = (int)(e >> 3);
+      slice = (int)(e & 7u);
+    }
     int nsl = 1;
     if (nmax > 1) {
-      const uint32_t longest = item_longest[J.item_off + blk];
-      nsl = (int)min((uint32_t)nmax, max(1u, longest / kSliceVisits));
+      const uint32_t per = cnt->slice_visits;
+      if (per)
+        nsl = (int)min((uint32_t)nmax,
+                       max(1u, item_longest[J.item_off + blk] / per));
     }
-    if (slice >= nsl) continue;
     const int i = point_of(J, blk, threadIdx.x);
     const uint32_t gi = (uint32_t)(J.ray_off + i);
     RayDir r;
@@ -944,20 +1042,6 @@ __device__ __forceinline__ float fold_small(float v, float a, uint32_t k) {
 // binade holds few events, events are summed one at a time).  Near zero, a
 // chunk with few nonzero gaps is summed as runs of hits instead: one fold
 // per run.  The result equals the sequential sum bit for bit.
-
-// Inclusive prefix sum over the 64 lanes of a fully active wavefront, in
-// DPP moves: lanes read lane - 1, 2, 4, 8 inside their 16-lane row (0 past
-// the row start), then lane 15 of each row is added to the next row (rows
-// 1 and 3) and lane 31 to rows 2 and 3.
-__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
-  return v;
-}
 
 // Increment of one event in units of the ulp (q = a / u exactly), or
 // false when it cannot be taken as a fixed step: larger than the binade,
@@ -1587,6 +1671,17 @@ hipError_t launch_discover(const Pipeline& p, const Workspace& b,
   return hipGetLastError();
 }
 
+
+int accumulate_resident(int cu_count) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&k_accumulate<kDown, 12>),
+          kCastThreads, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 6;
+  return per_cu * cu_count;
+}
+
 hipError_t launch_item_order(Pipeline& p, const Workspace& b, hipStream_t s) {
   p.accum_order = nullptr;
   if (p.tuning.queue != 3 || p.nitems <= 0) return hipSuccess;
@@ -1606,8 +1701,13 @@ hipError_t launch_compact_leaves(const Pipeline& p, const Workspace& b,
   const int nblocks = (int)((p.cube_vol + kScanChunk - 1) / kScanChunk);
   if (nblocks <= 0) return hipSuccess;
   k_scan_count<<<nblocks, 256, 0, s>>>(b.leaf_slot, p.cube_vol, b.scan_blocks);
-  k_scan_blocks<<<1, 1024, 0, s>>>(b.scan_blocks, nblocks, b.part_discover,
-                                   p.nitems, b.counters);
+  // a sliced single-job launch also plans its accumulate slices here
+  const bool sliced = p.njobs == 1 && p.accum_items > p.nitems;
+  k_scan_blocks<<<1, 1024, 0, s>>>(
+      b.scan_blocks, nblocks, b.part_discover, p.nitems, b.counters,
+      sliced ? b.item_longest : nullptr,
+      sliced ? p.host_jobs[0].slices : 1, p.accum_items - p.nitems,
+      b.slice_items);
   k_scan_assign<<<nblocks, 256, 0, s>>>(b.leaf_slot, p.cube_vol, b.scan_blocks,
                                         b.scan_job, b.slot_leaf, b.slot_job,
                                         p.tile_cap, b.counters);
@@ -1643,7 +1743,7 @@ void accumulate_as(const Pipeline& p, const Workspace& b, int sorted_buf,
                    const ItemQueue& q, int grid, hipStream_t s) {
   k_accumulate<M, S><<<grid, kCastThreads, 0, s>>>(
       b.jobs, q, b.leaf_slot, b.tiles, b.hit_mask, b.hit_start,
-      b.vals[sorted_buf], b.gap, b.counters, b.item_longest);
+      b.vals[sorted_buf], b.gap, b.counters, b.item_longest, b.slice_items);
 }
 
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,

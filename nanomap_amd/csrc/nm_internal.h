@@ -31,6 +31,9 @@ struct FrameCounters {
   int32_t n_filter_rays;    // rays emitted by the voxel filter
   int32_t n_filter_points;  // points binned by the voxel filter
   int32_t queue[4];         // work-queue heads of the persistent kernels
+  uint32_t slice_visits;    // sliced accumulate: visits per time slice
+                            // (k_slice_plan; 0 = no slicing)
+  int32_t slice_extra;      // sliced accumulate: entries of slice_items
 };
 
 // Statistics of one job, reduced on demand from per-item and per-slot
@@ -140,9 +143,8 @@ struct Job {
   int map;                     // index into the map table
   int depth;                   // 1: `cloud` is a depth image
   int slices;                  // accumulate work per ray block: up to
-                               // this many time slices (one work item each,
-                               // as many as the block's longest ray needs:
-                               // one per kSliceVisits visits); single-job
+                               // this many time slices (<= kMaxSlices), as
+                               // the slice plan decides; single-job
                                // launches only
   // placement in the shared workspace
   int64_t cube_off;  // first entry of the job's cube table (kScanChunk-aligned)
@@ -217,6 +219,8 @@ struct Workspace {
   int32_t* item_order;  // [item_cap]  queue order of the work items
   uint32_t* item_longest;  // [item_cap]  visits of each item's longest ray
                            //   (sliced accumulate)
+  uint32_t* slice_items;   // [item_cap * (kMaxSlices - 1)]  the extra
+                           //   accumulate items: block << 3 | slice
   uint32_t* cost_keys[2];  // [item_cap]  item cost keys (queue 3) and the
   uint32_t* cost_vals[2];  //             item indices sorted by them
   unsigned char* cloud; // staging for host clouds
@@ -271,7 +275,7 @@ struct Pipeline {
   int64_t n_rays;     // points of all jobs
   int nitems;         // cast work items of all jobs
   int accum_items = 0;  // work items of the accumulate pass (0: nitems; a
-                        // sliced job has nitems * slices)
+                        // sliced job adds the slice plan's extra items)
   int tile_cap;
   int sort_bits;
   uint32_t heavy_hits = kFrameHeavyHits;  // see kFrameHeavyHits
@@ -304,6 +308,13 @@ hipError_t launch_hit_segments(const Pipeline& p, const Workspace& b,
                                int sorted_buf, hipStream_t s);
 hipError_t launch_accumulate(const Pipeline& p, const Workspace& b,
                              int sorted_buf, hipStream_t s);
+// Sliced accumulate (single-job launches with accum_items > nitems): the
+// compact pass also picks the visits per time slice so that the extra
+// slice items fit in the accum_items - nitems extra work items, and lists
+// them (block, slice).
+constexpr int kMaxSlices = 8;  // time slices per ray block, at most
+// Accumulate workgroups resident on the current device at once.
+int accumulate_resident(int cu_count);
 // Queue 3: sorts the items by the cost keys the discover pass wrote and
 // points p.accum_order at the result.
 hipError_t launch_item_order(Pipeline& p, const Workspace& b, hipStream_t s);

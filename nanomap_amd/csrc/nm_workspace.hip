@@ -51,6 +51,7 @@ hipError_t workspace_alloc(const WorkspaceDims& d, Workspace* out) {
   A(dalloc(&b.item_job, I));
   A(dalloc(&b.item_order, I));
   A(dalloc(&b.item_longest, I));
+  A(dalloc(&b.slice_items, I * (kMaxSlices - 1)));
   A(dalloc(&b.cost_keys[0], I));
   A(dalloc(&b.cost_keys[1], I));
   A(dalloc(&b.cost_vals[0], I));
@@ -90,6 +91,7 @@ void workspace_free(Workspace* b) {
                   b->scan_blocks, b->scan_job,   b->counters,   b->job_stats,
                   b->jobs,       b->maps,        b->part_discover,
                   b->item_job,   b->item_order,  b->item_longest,
+                  b->slice_items,
                   b->cost_keys[0],
                   b->cost_keys[1], b->cost_vals[0], b->cost_vals[1],
                   b->cloud,      b->filter_rays,

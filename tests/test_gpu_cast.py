@@ -119,6 +119,35 @@ def test_hall_hundred_frames_bit_exact(gpu, tmp_path):
     _check_maps(restored, omap)
 
 
+def test_sensor_sizes_bit_exact(gpu):
+    """Frames of the 32-agent sensor mix, 160x120 to 1280x720 at 0.05 to
+    0.1 m, through one context in turn: small frames have room on the GPU
+    for the accumulate pass's time slices of their long-ray blocks, the
+    1280x720 frame fills it with its own blocks (no slices), 848x480 sits
+    between.  Every frame must give the oracle's counts and map."""
+    nm = gpu
+    _, agents = W.agents_32(nm)
+    picks = []
+    for want in [(160, 120, 0.05), (1280, 720, 0.1), (320, 240, 0.05),
+                 (848, 480, 0.05), (160, 120, 0.1), (640, 480, 0.05)]:
+        a = next(a for a in agents
+                 if (a["width"], a["height"], a["res"]) == want)
+        picks.append(a)
+    frames = [W.agent_frame(nm, a, RD) for a in picks]
+    ctx = nm.Context(max(len(a["cloud"]) for a in picks),
+                     max(f.max_time for f in frames) + 1.0)
+    for a, f in zip(picks, frames):
+        print("sensor %dx%d res %g range %.1f" % (a["width"], a["height"],
+                                                  a["res"], a["range"]),
+              flush=True)
+        gmap = nm.Map(a["res"], *W.CLAMP)
+        omap = oracle.Map(*W.CLAMP)
+        _run_frame(nm, ctx, gmap, omap, f, a["cloud"])
+        assert ctx.stats()["errors"] == 0
+        _check_maps(gmap, omap)
+        gmap.close()
+
+
 def test_edge_cases(gpu):
     nm = gpu
     I = np.eye(3, dtype=np.float32)
