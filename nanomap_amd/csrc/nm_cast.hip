@@ -697,10 +697,11 @@ template <int S>
 __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
                                           uint32_t* tiles, uint32_t* gap,
                                           uint32_t cell, uint32_t inc) {
-  // multiplicative hash on 24 bits (a full-rate multiply): the key's bits
-  // from 24 up are folded in first; bits [32-S, 32) of the product depend on
-  // every bit of the folded key
-  uint32_t h = (uint32_t)__umul24(cell ^ (cell >> 12), 0x9e3779u) >> (32 - S);
+  // multiplicative hash of the key's low 24 bits (a full-rate multiply):
+  // two of the few thousand cells of one work item rarely agree in those
+  // bits, and when they do they only share a probe sequence (the table
+  // compares whole keys)
+  uint32_t h = (uint32_t)__umul24(cell, 0x9e3779u) >> (32 - S);
   int found = -1;
 #pragma unroll
   for (int q = 0; q < kProbe; ++q) {
@@ -730,32 +731,35 @@ __device__ __forceinline__ void lds_count(uint32_t* s_key, uint32_t* s_cnt,
 // mostly stand in the same voxel (an 8x8 pixel wavefront covers a few
 // voxels at any depth), so lanes are merged before the table is touched:
 // when they all name the same voxel (the voxels next to the eye) one lane
-// adds the lane count; otherwise each run of equal cells along an 8-lane
-// pixel row (lane - 1 read across the row with a DPP shift) is added once,
-// by its first lane, with the run's length.
+// adds the lane count; otherwise each run of equal cells over consecutive
+// active lanes of a 16-lane row (two 8-pixel rows of an 8x8 part, one of a
+// 16x4 part; lane - 1 read with a DPP shift) is added once, by its first
+// lane, with the run's length.  prev_bit is 1 << (lane - 1), or 0 for the
+// first lane of a 16-lane row, which has no left neighbour.
 template <int S>
 __device__ __forceinline__ void wave_lds_count(uint32_t* s_key,
                                                uint32_t* s_cnt,
                                                uint32_t* tiles,
                                                uint32_t* gap,
-                                               uint32_t cell) {
+                                               uint32_t cell,
+                                               uint64_t prev_bit) {
   const uint64_t act = __ballot(1);
   const int lane = threadIdx.x & 63;
   const uint32_t first = __builtin_amdgcn_readfirstlane(cell);
-  const uint64_t same = __ballot(cell == first);
+  const uint64_t same = __builtin_amdgcn_ballot_w64(cell == first);
   if (same == act) {
     if (lane == (int)__builtin_ctzll(act))
       lds_count<S>(s_key, s_cnt, tiles, gap, cell, (uint32_t)__popcll(act));
     return;
   }
   // row_shr:1 -- lane l reads lane l - 1 of its 16-lane row
-  const uint32_t left =
-      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)cell, 0x111, 0xf, 0xf, false);
-  const bool left_act = (lane & 7) != 0 && ((act >> (lane - 1)) & 1ull);
-  const bool head = !left_act || left != cell;
-  const uint64_t stop = __ballot(head) | ~act | 0x0101010101010101ull;
+  const uint32_t left = (uint32_t)__builtin_amdgcn_update_dpp(
+      (int)kInvalid, (int)cell, 0x111, 0xf, 0xf, false);
+  const bool head = (act & prev_bit) == 0ull || left != cell;
+  const uint64_t stop =
+      __builtin_amdgcn_ballot_w64(head) | ~act | 0x0001000100010001ull;
   if (head) {
-    // a run ends at the next stop bit above the lane, which is at most 8
+    // a run ends at the next stop bit above the lane, which is at most 16
     // lanes up (every row start is a stop), so 32 bits of the shifted mask
     // suffice; past lane 63 there is none, which min(., 63 - lane) covers
     // (lane 63's shift by 0 included)
@@ -832,24 +836,96 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
         if (slice > 0) dda_advance<M>(s, r.L * slice / nsl);
         if (slice < nsl - 1) L = r.L * (slice + 1) / nsl;
       }
-      int px = s.vx, py = s.vy, pz = s.vz;
-      int entry = slot_of(J.cube, ls, px, py, pz);
+      // Walk state beside the DDA, kept so that a step that stays in its
+      // leaf costs two adds:
+      //   cell  slot * 512 + local index of the current voxel; a step along
+      //         an axis adds ci (+-64, +-8, +-1) unless it leaves the leaf;
+      //   U     per axis (4-bit fields x, y, z), the voxel's place in its
+      //         leaf counted along the axis's step direction (0..7): a step
+      //         adds 1 to the field, and a field reaching 8 has left the
+      //         leaf;
+      //   c     the cube-table index of the current leaf, moved by +-side^2,
+      //         +-side or +-1 as the walk enters the next leaf.
+      // A ray stays inside its frame's cube (the cube reaches a leaf and
+      // two voxels beyond max_time > L around the eye, and the walk stops
+      // at L), so c always names the leaf that holds the voxel; a c outside
+      // the table reads no entry (and counts errors, as an absent leaf
+      // does).
+      const int side = J.cube.side;
+      const int lx = (s.vx >> 3) - J.cube.cmin[0];
+      const int ly = (s.vy >> 3) - J.cube.cmin[1];
+      const int lz = (s.vz >> 3) - J.cube.cmin[2];
+      const uint32_t vol = (uint32_t)(side * side * side);
+      int c = (lx * side + ly) * side + lz;
+      if ((unsigned)lx >= (unsigned)side || (unsigned)ly >= (unsigned)side ||
+          (unsigned)lz >= (unsigned)side)
+        c = -1;
+      // place of v among 0..7 counted along step direction sgn (0 for an
+      // axis that never steps)
+      const auto place = [](int sgn, int v) {
+        return (uint32_t)(sgn > 0 ? (v & 7) : (sgn < 0 ? 7 - (v & 7) : 0));
+      };
+      uint32_t U = (place(s.sx, s.vx) << 8) | (place(s.sy, s.vy) << 4) |
+                   place(s.sz, s.vz);
+      const int cix = s.sx * 64, ciy = s.sy * 8, ciz = s.sz;
+      const int csx = s.sx * side * side, csy = s.sy * side, csz = s.sz;
+      // entries are read at a 32-bit byte offset (vol < 2^30, so 4 c fits)
+      const char* __restrict__ lsb = reinterpret_cast<const char*>(ls);
+      const auto entry_at = [lsb](uint32_t k) {
+        return *reinterpret_cast<const int32_t*>(lsb + k * 4u);
+      };
+      int entry = -1;
+      if ((uint32_t)c < vol) entry = entry_at((uint32_t)c);
+      uint32_t cell = ((uint32_t)(entry & ~kSlotHasHits) << 9) |
+                      (uint32_t)local_of(s.vx, s.vy, s.vz);
+      const int lane = threadIdx.x & 63;
+      const uint64_t prev_bit = (lane & 15) ? 1ull << (lane - 1) : 0ull;
       if (M == kDown) fp64_round_down_begin(s, L);
-      while (dda_step<M, 2, M == kDown>(s, L)) {
-        // miss on (px, py, pz)
+      // The loop is software-pipelined around its two memory reads: the
+      // hit-mask word of the current voxel and the next leaf's entry are
+      // requested before the table work for the current voxel and used
+      // after it, so their latency overlaps that work.
+      // the DDA step's axis (dda_step, nested-select form): x only if
+      // strictly smallest, else y if y < z, else z; t its crossing time
+      bool bx, by;
+      double t;
+      {
+        const double nx = s.nx, ny = s.ny, nz = s.nz;
+        bx = (nx < ny) && (nx < nz);
+        by = !bx && (ny < nz);
+        t = bx ? nx : (by ? ny : nz);
+      }
+      while (t <= L) {
+        // a miss on the current voxel (its exit time t <= L)
+        uint32_t mw = 0;
+        if (entry >= 0 && (entry & kSlotHasHits))
+          mw = hit_mask[cell >> 5];  // kMaskWords words per slot
+        // the step: next crossing time of the axis (round-down: the wave's
+        // FP64 mode is toward -inf here, so this is the rounded-down sum)
+        const double dt = bx ? s.dx : (by ? s.dy : s.dz);
+        const double nt = t + dt;
+        s.nx = bx ? nt : s.nx;
+        s.ny = by ? nt : s.ny;
+        s.nz = (!bx && !by) ? nt : s.nz;
+        // the voxel it moves to; leaving the leaf, the axis's place starts
+        // again at 0 (the one field at 8 drops to 0) and the next leaf's
+        // entry is read
+        const int ci = bx ? cix : (by ? ciy : ciz);
+        const uint32_t ui = bx ? 0x100u : (by ? 0x10u : 0x1u);
+        U += ui;
+        const bool cross = (U & 0x888u) != 0u;
+        int next_entry = entry;
+        if (cross) {
+          U &= ~0x888u;
+          c += bx ? csx : (by ? csy : csz);
+          next_entry = -1;
+          if ((uint32_t)c < vol) next_entry = entry_at((uint32_t)c);
+        }
         if (entry < 0) {
           ++errors;
         } else {
-          const int slot = entry & ~kSlotHasHits;
-          const int li = local_of(px, py, pz);
-          const uint32_t cell = (uint32_t)slot * 512u + (uint32_t)li;
-          bool hit_voxel = false;
-          if (entry & kSlotHasHits) {
-            const uint32_t mw = hit_mask[slot * kMaskWords + (li >> 5)];
-            hit_voxel = (mw >> (li & 31)) & 1u;
-          }
-          wave_lds_count<S>(s_key, s_cnt, tiles, gap, cell);
-          if (hit_voxel) {
+          wave_lds_count<S>(s_key, s_cnt, tiles, gap, cell, prev_bit);
+          if ((mw >> (cell & 31)) & 1u) {
             // a voxel holding hits also counts the miss in the gap before
             // its first later hit (the voxel's hits are sorted[s0, end) in
             // ascending point order); neighbouring rays mostly share the
@@ -865,14 +941,24 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
                 hi = mid;
             }
             if (lo < end)
-              wave_lds_count<S>(s_key, s_cnt, tiles, gap, kGapKey | lo);
+              wave_lds_count<S>(s_key, s_cnt, tiles, gap, kGapKey | lo,
+                                prev_bit);
           }
         }
-        if (((s.vx ^ px) | (s.vy ^ py) | (s.vz ^ pz)) & ~7)
-          entry = slot_of(J.cube, ls, s.vx, s.vy, s.vz);
-        px = s.vx;
-        py = s.vy;
-        pz = s.vz;
+        // the local index wraps to the far side of the next leaf (li + ci -
+        // 8 ci) when the step leaves the leaf
+        if (cross)
+          cell = ((uint32_t)(next_entry & ~kSlotHasHits) << 9) |
+                 ((cell & 511u) - 7u * (uint32_t)ci);
+        else
+          cell += (uint32_t)ci;
+        entry = next_entry;
+        {
+          const double nx = s.nx, ny = s.ny, nz = s.nz;
+          bx = (nx < ny) && (nx < nz);
+          by = !bx && (ny < nz);
+          t = bx ? nx : (by ? ny : nz);
+        }
       }
       if (M == kDown) fp64_round_nearest();
     }
