@@ -34,6 +34,8 @@ constexpr int kThreads = 256;
 constexpr int kMinItems = kRadixTile / kThreads;  // 8
 constexpr int kMaxItems = 64;
 constexpr int kWaves = kThreads / 64;
+constexpr int kChunk = 8;  // iterations whose loads are issued together
+static_assert(kMinItems % kChunk == 0, "items must be a multiple of kChunk");
 
 // Lanes of the wavefront holding the same digit as this one (D ballots).
 template <int D>
@@ -61,17 +63,29 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(
   for (int b = tid; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
   const int base = blockIdx.x * items * kThreads;
-  for (int j = 0; j < items; ++j) {
-    const int i = base + j * kThreads + tid;
-    const uint32_t key = i < n ? keys[i] : kInvalid;
-    const bool valid = i < n && !(drop && key == kInvalid);
-    const uint32_t d = valid ? (key >> shift) & (kBins - 1) : 0u;
-    if (match) {
-      const uint64_t peers = digit_peers<D>(d, valid);
-      if (valid && (peers & lt) == 0)
-        atomicAdd(&h[d], (uint32_t)__popcll(peers));
-    } else if (valid) {
-      atomicAdd(&h[d], 1u);
+  // the keys of kChunk iterations are requested together (items is a
+  // multiple of kChunk), so their loads overlap instead of each iteration
+  // waiting for its own
+  for (int j0 = 0; j0 < items; j0 += kChunk) {
+    uint32_t kk[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int i = base + (j0 + u) * kThreads + tid;
+      kk[u] = i < n ? keys[i] : kInvalid;
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int i = base + (j0 + u) * kThreads + tid;
+      const uint32_t key = kk[u];
+      const bool valid = i < n && !(drop && key == kInvalid);
+      const uint32_t d = valid ? (key >> shift) & (kBins - 1) : 0u;
+      if (match) {
+        const uint64_t peers = digit_peers<D>(d, valid);
+        if (valid && (peers & lt) == 0)
+          atomicAdd(&h[d], (uint32_t)__popcll(peers));
+      } else if (valid) {
+        atomicAdd(&h[d], 1u);
+      }
     }
   }
   __syncthreads();
@@ -161,34 +175,43 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int base = blockIdx.x * items * kThreads;
-  for (int j = 0; j < items; ++j) {
-    const int i = base + j * kThreads + tid;
-    uint32_t key = i < n ? kin[i] : kInvalid;
-    const bool valid = i < n && !(drop && key == kInvalid);
-    uint32_t val = 0, d = 0;
-    if (valid) {
-      val = vin[i];
-      d = (key >> shift) & (kBins - 1);
+  // keys and values of kChunk iterations are requested together (items is
+  // a multiple of kChunk); the iterations then run in order, so the ranks
+  // stay stable
+  for (int j0 = 0; j0 < items; j0 += kChunk) {
+    uint32_t kk[kChunk], vv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int i = base + (j0 + u) * kThreads + tid;
+      kk[u] = i < n ? kin[i] : kInvalid;
+      vv[u] = i < n ? vin[i] : 0u;
     }
-    const uint64_t peers = digit_peers<D>(d, valid);
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    const uint32_t cnt = (uint32_t)__popcll(peers);
-    const bool leader = valid && rank == 0;
-    if (leader) wcnt[wave][d] = cnt;
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = offset[d] + running[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    // every lane has read: the leaders clear their wave's count and move
-    // the digit's running count on (ordered before the next reads by the
-    // next iteration's barrier)
-    if (leader) {
-      wcnt[wave][d] = 0;
-      atomicAdd(&running[d], cnt);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int i = base + (j0 + u) * kThreads + tid;
+      const uint32_t key = kk[u];
+      const bool valid = i < n && !(drop && key == kInvalid);
+      const uint32_t d = valid ? (key >> shift) & (kBins - 1) : 0u;
+      const uint64_t peers = digit_peers<D>(d, valid);
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      const uint32_t cnt = (uint32_t)__popcll(peers);
+      const bool leader = valid && rank == 0;
+      if (leader) wcnt[wave][d] = cnt;
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = offset[d] + running[d] + rank;
+        for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+        kout[pos] = key;
+        vout[pos] = vv[u];
+      }
+      __syncthreads();
+      // every lane has read: the leaders clear their wave's count and move
+      // the digit's running count on (ordered before the next reads by the
+      // next iteration's barrier)
+      if (leader) {
+        wcnt[wave][d] = 0;
+        atomicAdd(&running[d], cnt);
+      }
     }
   }
 }
