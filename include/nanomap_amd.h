@@ -195,6 +195,40 @@ int nm_map_save(nm_map* map, const char* path);
 int nm_map_load(int device, const char* path, int64_t leaf_capacity,
                 nm_map** out);
 
+/* NanoVDB export and import.  The reference hands its map to the GPU as a
+ * NanoVDB grid (nanovdb::openToNanoVDB, src/nanomap/handler/
+ * SimHandler.cpp:22-23) and its simulator reads grids from files
+ * (SimManager.h:196-210).  These write a map's leaves as a NanoVDB 32.4
+ * FloatGrid (breadth-first root/upper/lower/leaf nodes, every voxel in a
+ * leaf, background 0, no checksum) and read FloatGrids back.
+ * nm_nvdb_build: the grid buffer of n host leaves (nm_map_download
+ *   layout) into `buffer` (*bytes in: its size, out: the grid's size; a
+ *   null buffer only returns the size).  Host only.
+ * nm_nvdb_write: the same grid as an uncompressed .nvdb file (16-byte
+ *   header, 176-byte grid record, name, grid).  Host only.
+ * nm_nvdb_read: the leaves of the FloatGrid named grid_name (the first
+ *   FloatGrid for "" or NULL) of an uncompressed .nvdb file: leaf nodes as
+ *   they are, active tiles expanded into leaves (at most 2^22 of them);
+ *   with null arrays *n_leaves returns the count, else the arrays hold
+ *   *n_leaves entries.  Host only.
+ * nm_map_write_nvdb: nm_map_download, then nm_nvdb_write.
+ * nm_map_load_nvdb: a map from nm_nvdb_read's leaves, with the grid's
+ *   voxel size and the given clamps and thresholds (a grid file carries
+ *   none); leaf_capacity as for nm_map_load. */
+int nm_nvdb_build(const int32_t* origins, const float* values,
+                  const uint32_t* masks, int64_t n_leaves, double voxel_size,
+                  const char* grid_name, void* buffer, uint64_t* bytes);
+int nm_nvdb_write(const char* path, const int32_t* origins,
+                  const float* values, const uint32_t* masks, int64_t n_leaves,
+                  double voxel_size, const char* grid_name);
+int nm_nvdb_read(const char* path, const char* grid_name, int64_t* n_leaves,
+                 int32_t* origins, float* values, uint32_t* masks,
+                 double* voxel_size);
+int nm_map_write_nvdb(nm_map* map, const char* path, const char* grid_name);
+int nm_map_load_nvdb(int device, const char* path, const char* grid_name,
+                     float clamp_min, float clamp_max, float hit_thr,
+                     float miss_thr, int64_t leaf_capacity, nm_map** out);
+
 /* ---- plain frame path --------------------------------------------------
  * nm_cast: filterCloud + frustumCastCloud (filterCloud.cu:476,
  * frustumCastCloud.cu:311).  `cloud` holds n_points xyz-f32 points with

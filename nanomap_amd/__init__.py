@@ -345,6 +345,10 @@ class Map:
         h = ctypes.c_void_p()
         check(lib().nm_map_load(device, str(path).encode(), int(leaf_capacity),
                                 ctypes.byref(h)))
+        return cls._wrap(h)
+
+    @classmethod
+    def _wrap(cls, h):
         m = cls.__new__(cls)
         m.handle = h
         res = ctypes.c_float()
@@ -352,6 +356,71 @@ class Map:
                                   None))
         m.grid_res = res.value
         return m
+
+    def write_nvdb(self, path, grid_name="occupancy"):
+        """The map as a NanoVDB FloatGrid file (nm_map_write_nvdb)."""
+        check(lib().nm_map_write_nvdb(self.handle, str(path).encode(),
+                                      grid_name.encode()))
+
+    @classmethod
+    def load_nvdb(cls, path, grid_name="", clamp_min=-2.0, clamp_max=3.5,
+                  hit_thr=0.0, miss_thr=0.0, leaf_capacity=0, device=0):
+        """A map from a FloatGrid of a .nvdb file (nm_map_load_nvdb)."""
+        h = ctypes.c_void_p()
+        check(lib().nm_map_load_nvdb(device, str(path).encode(),
+                                     grid_name.encode(), clamp_min, clamp_max,
+                                     hit_thr, miss_thr, int(leaf_capacity),
+                                     ctypes.byref(h)))
+        return cls._wrap(h)
+
+
+def _leaf_arrays(origins, values, masks):
+    o = np.ascontiguousarray(origins, np.int32).reshape(-1, 3)
+    v = np.ascontiguousarray(values, np.float32).reshape(-1, 512)
+    k = np.ascontiguousarray(masks, np.uint32).reshape(-1, 16)
+    if not (len(o) == len(v) == len(k)):
+        raise ValueError("origins, values and masks need one row per leaf")
+    return o, v, k
+
+
+def nvdb_build(origins, values, masks, voxel_size, grid_name="occupancy"):
+    """NanoVDB FloatGrid buffer (bytes) of host leaves (nm_nvdb_build)."""
+    o, v, k = _leaf_arrays(origins, values, masks)
+    n = ctypes.c_uint64(0)
+    args = (o.ctypes.data, v.ctypes.data, k.ctypes.data, len(o),
+            float(voxel_size), grid_name.encode())
+    check(lib().nm_nvdb_build(*args, None, ctypes.byref(n)))
+    buf = (ctypes.c_ubyte * n.value)()
+    check(lib().nm_nvdb_build(*args, buf, ctypes.byref(n)))
+    return bytes(buf)
+
+
+def nvdb_write(path, origins, values, masks, voxel_size,
+               grid_name="occupancy"):
+    """Host leaves as a .nvdb file (nm_nvdb_write)."""
+    o, v, k = _leaf_arrays(origins, values, masks)
+    check(lib().nm_nvdb_write(str(path).encode(), o.ctypes.data,
+                              v.ctypes.data, k.ctypes.data, len(o),
+                              float(voxel_size), grid_name.encode()))
+
+
+def nvdb_read(path, grid_name=""):
+    """(origins, values, masks, voxel_size) of a .nvdb FloatGrid
+    (nm_nvdb_read)."""
+    n = ctypes.c_int64(0)
+    dx = ctypes.c_double(0)
+    check(lib().nm_nvdb_read(str(path).encode(), grid_name.encode(),
+                             ctypes.byref(n), None, None, None,
+                             ctypes.byref(dx)))
+    o = np.zeros((n.value, 3), np.int32)
+    v = np.zeros((n.value, 512), np.float32)
+    k = np.zeros((n.value, 16), np.uint32)
+    if n.value:
+        check(lib().nm_nvdb_read(str(path).encode(), grid_name.encode(),
+                                 ctypes.byref(n), o.ctypes.data,
+                                 v.ctypes.data, k.ctypes.data,
+                                 ctypes.byref(dx)))
+    return o, v, k, dx.value
 
 
 class Batch:

@@ -159,6 +159,18 @@ SIGNATURES = {
     "nm_map_query": (_I, [_P, _P, ctypes.c_int64, _P, _P, _I, _P]),
     "nm_map_dense": (_I, [_P, _P, _I, _I, _I, ctypes.c_float, _P, _P]),
     "nm_map_save": (_I, [_P, ctypes.c_char_p]),
+    "nm_nvdb_build": (_I, [_P, _P, _P, ctypes.c_int64, ctypes.c_double,
+                           ctypes.c_char_p, _P,
+                           ctypes.POINTER(ctypes.c_uint64)]),
+    "nm_nvdb_write": (_I, [ctypes.c_char_p, _P, _P, _P, ctypes.c_int64,
+                           ctypes.c_double, ctypes.c_char_p]),
+    "nm_nvdb_read": (_I, [ctypes.c_char_p, ctypes.c_char_p,
+                          ctypes.POINTER(ctypes.c_int64), _P, _P, _P,
+                          ctypes.POINTER(ctypes.c_double)]),
+    "nm_map_write_nvdb": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p]),
+    "nm_map_load_nvdb": (_I, [_I, ctypes.c_char_p, ctypes.c_char_p,
+                              ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_int64, _P]),
     "nm_map_load": (_I, [_I, ctypes.c_char_p, ctypes.c_int64,
                          ctypes.POINTER(_P)]),
     "nm_cast": (_I, [_P, ctypes.POINTER(Frame), _P, _I, _I, _I, _P]),

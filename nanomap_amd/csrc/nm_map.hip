@@ -332,6 +332,47 @@ struct SnapHeader {
 };
 static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
 
+// A map on `device` holding n host leaves (nm_map_download layout), with
+// room for max(leaf_capacity, n) leaves, or for max(2 n, 2^18) with
+// leaf_capacity 0.
+int map_from_leaves(int device, float grid_res, float clamp_min,
+                    float clamp_max, float hit_thr, float miss_thr, int64_t n,
+                    const int32_t* o, const float* v, const uint32_t* k,
+                    int64_t leaf_capacity, nm_map** out, const char* who) {
+  const int64_t cap = leaf_capacity > 0
+                          ? std::max<int64_t>(leaf_capacity, std::max<int64_t>(n, 1))
+                          : std::max<int64_t>(2 * n, (int64_t)1 << 18);
+  if (cap > (INT_MAX - 64) / 512)
+    return fail(NM_EINVAL, std::string(who) + ": capacity too large");
+  nm_map* m = nullptr;
+  int st = nm_map_create(device, grid_res, clamp_min, clamp_max, hit_thr,
+                         miss_thr, (int)cap, &m);
+  if (st != NM_OK) return st;
+  hipError_t e = hipSuccess;
+  if (n) {
+    e = hipMemcpy(m->v.origins, o, (size_t)n * 12, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(m->v.values, v, (size_t)n * 2048, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(m->v.masks, k, (size_t)n * nm::kMaskWords * 4,
+                    hipMemcpyHostToDevice);
+  }
+  const int32_t cnt[2] = {(int32_t)n, 0};
+  if (e == hipSuccess)
+    e = hipMemcpy(m->v.leaf_count, cnt, sizeof(cnt), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    nm_map_destroy(m);
+    return hip_fail(e, who);
+  }
+  st = nm::rehash(m->v, (int)n);
+  if (st != NM_OK) {
+    nm_map_destroy(m);
+    return st;
+  }
+  *out = m;
+  return NM_OK;
+}
+
 }  // namespace
 
 int nm_map_save(nm_map* m, const char* path) {
@@ -392,40 +433,47 @@ int nm_map_load(int device, const char* path, int64_t leaf_capacity,
   }
   fclose(f);
   if (!ok) return fail(NM_EINVAL, std::string("nm_map_load: not a valid map snapshot: ") + path);
-  // 0: room for the file's leaves and as many again (at least 2^18, the
-  // default of a new map); otherwise at least the file's leaves
-  const int64_t cap = leaf_capacity > 0
-                          ? std::max<int64_t>(leaf_capacity, std::max<int64_t>(n, 1))
-                          : std::max<int64_t>(2 * n, (int64_t)1 << 18);
-  if (cap > (INT_MAX - 64) / 512)
-    return fail(NM_EINVAL, "nm_map_load: capacity too large");
-  nm_map* m = nullptr;
-  int st = nm_map_create(device, h.grid_res, h.clamp_min, h.clamp_max,
-                         h.hit_thr, h.miss_thr, (int)cap, &m);
+  return map_from_leaves(device, h.grid_res, h.clamp_min, h.clamp_max,
+                         h.hit_thr, h.miss_thr, n, o.data(), v.data(),
+                         k.data(), leaf_capacity, out, "nm_map_load");
+}
+
+int nm_map_write_nvdb(nm_map* m, const char* path, const char* grid_name) {
+  if (!m || !path || !grid_name)
+    return fail(NM_EINVAL, "nm_map_write_nvdb: null argument");
+  int64_t n = 0;
+  int st = nm_map_leaf_count(m, &n);
   if (st != NM_OK) return st;
-  hipError_t e = hipSuccess;
-  if (n) {
-    e = hipMemcpy(m->v.origins, o.data(), (size_t)n * 12, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(m->v.values, v.data(), (size_t)n * 2048, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(m->v.masks, k.data(), (size_t)n * nm::kMaskWords * 4,
-                    hipMemcpyHostToDevice);
-  }
-  const int32_t cnt[2] = {(int32_t)n, 0};
-  if (e == hipSuccess)
-    e = hipMemcpy(m->v.leaf_count, cnt, sizeof(cnt), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    nm_map_destroy(m);
-    return hip_fail(e, "nm_map_load");
-  }
-  st = nm::rehash(m->v, (int)n);
-  if (st != NM_OK) {
-    nm_map_destroy(m);
-    return st;
-  }
-  *out = m;
-  return NM_OK;
+  std::vector<int32_t> o((size_t)std::max<int64_t>(n, 1) * 3);
+  std::vector<float> v((size_t)std::max<int64_t>(n, 1) * 512);
+  std::vector<uint32_t> k((size_t)std::max<int64_t>(n, 1) * nm::kMaskWords);
+  st = nm_map_download(m, o.data(), v.data(), k.data());
+  if (st != NM_OK) return st;
+  return nm_nvdb_write(path, o.data(), v.data(), k.data(), n,
+                       (double)m->grid_res, grid_name);
+}
+
+int nm_map_load_nvdb(int device, const char* path, const char* grid_name,
+                     float clamp_min, float clamp_max, float hit_thr,
+                     float miss_thr, int64_t leaf_capacity, nm_map** out) {
+  if (!path || !out) return fail(NM_EINVAL, "nm_map_load_nvdb: null argument");
+  *out = nullptr;
+  int64_t n = 0;
+  double dx = 0;
+  int st = nm_nvdb_read(path, grid_name, &n, nullptr, nullptr, nullptr, &dx);
+  if (st != NM_OK) return st;
+  if (n > (INT_MAX - 64) / 512)
+    return fail(NM_EINVAL, "nm_map_load_nvdb: too many leaves");
+  std::vector<int32_t> o((size_t)std::max<int64_t>(n, 1) * 3);
+  std::vector<float> v((size_t)std::max<int64_t>(n, 1) * 512);
+  std::vector<uint32_t> k((size_t)std::max<int64_t>(n, 1) * nm::kMaskWords);
+  st = nm_nvdb_read(path, grid_name, &n, o.data(), v.data(), k.data(), &dx);
+  if (st != NM_OK) return st;
+  if (!(dx > 0) || !(dx < 1e30))
+    return fail(NM_EINVAL, "nm_map_load_nvdb: the grid's voxel size is not positive");
+  return map_from_leaves(device, (float)dx, clamp_min, clamp_max, hit_thr,
+                         miss_thr, n, o.data(), v.data(), k.data(),
+                         leaf_capacity, out, "nm_map_load_nvdb");
 }
 
 }  // extern "C"

@@ -10,10 +10,11 @@
 // observation rays, getRandomSpawn() draws a free spawn pose.
 //
 // Differences from the reference, each on purpose:
-//  - the grid is a map snapshot (nm_map_save; "SimGrid:" in the config) or
-//    a map in memory, not an OpenVDB .vdb file (OpenVDB is not a
-//    dependency); its voxels with log-odds above the map's hit threshold
-//    are the occupied ones (the reference's active voxels);
+//  - the grid is a map snapshot (nm_map_save; "SimGrid:" in the config), a
+//    NanoVDB FloatGrid file (.nvdb, nm_map_load_nvdb) or a map in memory,
+//    not an OpenVDB .vdb file (OpenVDB is not a dependency); its voxels
+//    with log-odds above the map's hit threshold are the occupied ones (the
+//    reference's active voxels);
 //  - updateAgentViews() simulates every agent's sensors in one launch
 //    (nm_sim_cast) and, for clouds cast without the voxel filter, folds
 //    each sensor round of all agents into their maps in one batch
@@ -47,14 +48,24 @@ namespace manager {
 
 class SimManager {
  public:
-  // The grid from the config's "SimGrid:" snapshot.
+  // The grid from the config's "SimGrid:" file: a map snapshot, or a
+  // NanoVDB FloatGrid (a .nvdb file).
   explicit SimManager(std::shared_ptr<config::Config> config, int device = 0)
       : _config(std::move(config)), _device(device), _manager(_config, device) {
     if (_config->simGrid().empty())
       throw std::runtime_error("SimManager: the config names no SimGrid");
-    nm_map* m = nullptr;
-    check(nm_map_load(device, _config->simGrid().c_str(), 0, &m));
-    _simMap.reset(m, nm_map_destroy);
+    const std::string& g = _config->simGrid();
+    if (g.size() > 5 && g.compare(g.size() - 5, 5, ".nvdb") == 0) {
+      // a NanoVDB FloatGrid (the first in the file), read with the map's
+      // clamps and thresholds
+      _simMapRef = std::make_shared<map::Map>(g, "", _config->probHitThres(),
+                                              _config->probMissThres(), device);
+      _simMap = std::shared_ptr<nm_map>(_simMapRef->handle(), [](nm_map*) {});
+    } else {
+      nm_map* m = nullptr;
+      check(nm_map_load(device, g.c_str(), 0, &m));
+      _simMap.reset(m, nm_map_destroy);
+    }
     init();
   }
   // The grid from a map in memory.

@@ -102,9 +102,33 @@ class Map {
                         &_occupiedClampingThreshold, &_logOddsHitThreshold,
                         &_logOddsMissThreshold));
   }
+  // A map from the FloatGrid `gridName` ("" for the first) of a NanoVDB
+  // file, with the clamps and thresholds of Map(gridRes, probThresHit,
+  // probThresMiss); the voxel size comes from the grid.
+  Map(const std::string& nvdbFile, const std::string& gridName,
+      float probThresHit, float probThresMiss, int device = 0) {
+    _logOddsMissThreshold =
+        (float)(std::log((double)probThresMiss) - std::log(1.0 - probThresMiss));
+    _logOddsHitThreshold =
+        (float)(std::log((double)probThresHit) - std::log(1.0 - probThresHit));
+    _occupiedClampingThreshold = (float)(std::log(0.97) - std::log(1 - 0.97));
+    _emptyClampingThreshold = (float)(std::log(0.12) - std::log(1 - 0.12));
+    check(nm_map_load_nvdb(device, nvdbFile.c_str(), gridName.c_str(),
+                           _emptyClampingThreshold, _occupiedClampingThreshold,
+                           _logOddsHitThreshold, _logOddsMissThreshold, 0,
+                           &_map));
+    check(nm_map_set_auto_grow(_map, 1));
+    check(nm_map_params(_map, &_gridRes, nullptr, nullptr, nullptr, nullptr));
+  }
   ~Map() { nm_map_destroy(_map); }
 
   void save(const std::string& path) const { check(nm_map_save(_map, path.c_str())); }
+  // The map as a NanoVDB FloatGrid file (the grid form the reference hands
+  // to its GPU kernels, SimHandler.cpp:22-23).
+  void writeNanoVDB(const std::string& path,
+                    const std::string& gridName = "occupancy") const {
+    check(nm_map_write_nvdb(_map, path.c_str(), gridName.c_str()));
+  }
   Map(const Map&) = delete;
   Map& operator=(const Map&) = delete;
 

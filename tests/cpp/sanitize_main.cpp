@@ -17,18 +17,26 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
+#include <iterator>
 #include <string>
 #include <vector>
 
 #include "../../include/nanomap_amd.h"
 #include "../../include/nanomap_amd_scene.h"
+
 #include "../../oracle/nm_oracle.h"
 #include "nanomap/config/Config.h"
 
 // nanomap::check() reports failures through the library's last-error
 // string; this binary links only nm_scene.cpp, whose calls return codes.
 extern "C" const char* nm_last_error(void) { return "nm_scene call failed"; }
+// nm_nvdb.cpp reports errors through the library's message slot, which
+// lives in the HIP part of the library
+namespace nm {
+void set_error(const std::string&) {}
+}
 
 static int failures = 0;
 #define EXPECT(c)                                                   \
@@ -211,11 +219,60 @@ static void config_files(const std::string& dir) {
   EXPECT(c->sharedParameters()._leafBufferSize > 0);
 }
 
+// NanoVDB export and import (csrc/nm_nvdb.cpp): a grid of leaves in
+// several root, upper and lower nodes written and read back, then damaged
+// files (truncated, child offsets out of the buffer) that must be refused.
+static void nanovdb_files(const std::string& dir) {
+  const int n = 5;
+  const int32_t orgs[3 * n] = {0, 0, 0, 8, 0, 0, 128, -8, 0, -4096, 4096, 64, 16, 16, 16};
+  std::vector<float> vals((size_t)n * 512);
+  std::vector<uint32_t> masks((size_t)n * 16);
+  for (size_t k = 0; k < vals.size(); ++k) vals[k] = std::sin(0.1 * (double)k);
+  for (size_t k = 0; k < masks.size(); ++k) masks[k] = (uint32_t)(k * 2654435761u);
+  uint64_t bytes = 0;
+  EXPECT(nm_nvdb_build(orgs, vals.data(), masks.data(), n, 0.1, "g", nullptr,
+                       &bytes) == NM_OK);
+  std::vector<unsigned char> buf(bytes);
+  EXPECT(nm_nvdb_build(orgs, vals.data(), masks.data(), n, 0.1, "g", buf.data(),
+                       &bytes) == NM_OK);
+  const std::string path = dir + "/sanitize.nvdb";
+  EXPECT(nm_nvdb_write(path.c_str(), orgs, vals.data(), masks.data(), n, 0.1,
+                       "g") == NM_OK);
+  int64_t got = 0;
+  double dx = 0;
+  EXPECT(nm_nvdb_read(path.c_str(), "g", &got, nullptr, nullptr, nullptr, &dx) == NM_OK);
+  EXPECT(got == n && dx == 0.1);
+  std::vector<int32_t> o((size_t)got * 3);
+  std::vector<float> v((size_t)got * 512);
+  std::vector<uint32_t> m((size_t)got * 16);
+  EXPECT(nm_nvdb_read(path.c_str(), "", &got, o.data(), v.data(), m.data(), &dx) == NM_OK);
+  std::ifstream in(path, std::ios::binary);
+  std::vector<char> file((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  const std::string cut = dir + "/sanitize_cut.nvdb";
+  std::ofstream(cut, std::ios::binary).write(file.data(), (std::streamsize)file.size() / 2);
+  EXPECT(nm_nvdb_read(cut.c_str(), "", &got, nullptr, nullptr, nullptr, &dx) != NM_OK);
+  // every 8-byte word of the root table and upper node header set to a
+  // large offset, one at a time
+  const size_t grid_at = 16 + 176 + 2;
+  for (size_t w = 0; w < 64; ++w) {
+    std::vector<char> bad = file;
+    const size_t at = grid_at + 736 + 8 * w;
+    if (at + 8 > bad.size()) break;
+    const int64_t big = (int64_t)1 << 40;
+    std::memcpy(bad.data() + at, &big, 8);
+    const std::string bp = dir + "/sanitize_bad.nvdb";
+    std::ofstream(bp, std::ios::binary).write(bad.data(), (std::streamsize)bad.size());
+    int64_t nb = 0;
+    (void)nm_nvdb_read(bp.c_str(), "", &nb, nullptr, nullptr, nullptr, &dx);
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string dir = argc > 1 ? argv[1] : ".";
   scene_and_oracle();
   soft_march();
   config_files(dir);
+  nanovdb_files(dir);
   std::printf("sanitize: %d failed checks\n", failures);
   return failures == 0 ? 0 : 1;
 }

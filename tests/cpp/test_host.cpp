@@ -54,6 +54,9 @@ static int parse(const char* file) {
   return 0;
 }
 
+static bool same_maps(const std::shared_ptr<nanomap::map::Map>& a,
+                      const std::shared_ptr<nanomap::map::Map>& b);
+
 static int map(int argc, char** argv) {
   auto cfg = std::make_shared<nanomap::config::Config>(argv[2]);
   const std::string sensor = argv[3];
@@ -117,6 +120,16 @@ static int map(int argc, char** argv) {
         return 5;
       }
     }
+  }
+  // the map as a NanoVDB grid file (<out>.nvdb), and a map read back from
+  // it holds the same leaves
+  const std::string nv = std::string(argv[6]) + ".nvdb";
+  m->writeNanoVDB(nv, "occupancy");
+  auto back = std::make_shared<nanomap::map::Map>(
+      nv, "occupancy", cfg->probHitThres(), cfg->probMissThres());
+  if (back->gridRes() != m->gridRes() || !same_maps(m, back)) {
+    std::fprintf(stderr, "NanoVDB round trip differs\n");
+    return 6;
   }
   std::ofstream out(argv[6], std::ios::binary);
   out.write(reinterpret_cast<const char*>(&n), 8);

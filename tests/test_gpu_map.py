@@ -134,3 +134,44 @@ def test_snapshot_restore_continues_exactly(gpu, tmp_path):
     bad.write_bytes(data[:len(data) // 2])
     with pytest.raises(nm.NanoMapError):
         nm.Map.load(bad)
+
+
+def test_nanovdb_export_and_load(gpu, tmp_path):
+    # the map of 8 hall frames written as a NanoVDB FloatGrid: every leaf
+    # (values, active masks) read back through the layout reader equals the
+    # device map; a map loaded from the file equals it too, and 4 more
+    # frames on both maps give the same map
+    import nvdb_layout as L
+    nm = gpu
+    cam, frames = _hall(nm, 12)
+    ctx = nm.Context(cam.width * cam.height, 200.0)
+    gmap = nm.Map(0.05, *W.CLAMP, leaf_capacity=1 << 16)
+    _feed(nm, ctx, gmap, None, frames[:8])
+    path = tmp_path / "hall.nvdb"
+    gmap.write_nvdb(path, "hall")
+    o, v, m = gmap.download()
+    _, grids = L.read_file(path.read_bytes())
+    rec, name, grid = grids[0]
+    assert name == "hall" and rec["voxel_size"] == (np.float32(0.05),) * 3
+    got = grid.leaves()
+    go = np.array([x[0] for x in got], np.int32)
+    gv = np.stack([x[1] for x in got])
+    gb = np.stack([x[2] for x in got])
+    bits = ((m[:, np.arange(512) >> 5] >> (np.arange(512) & 31)) & 1).astype(bool)
+    i = np.lexsort((o[:, 2], o[:, 1], o[:, 0]))
+    j = np.lexsort((go[:, 2], go[:, 1], go[:, 0]))
+    assert np.array_equal(o[i], go[j])
+    assert np.array_equal(v[i].view(np.uint32), gv[j].view(np.uint32))
+    assert np.array_equal(bits[i], gb[j])
+    loaded = nm.Map.load_nvdb(path, "hall", *W.CLAMP)
+    lo, lv, lm = loaded.download()
+    k = np.lexsort((lo[:, 2], lo[:, 1], lo[:, 0]))
+    assert np.array_equal(o[i], lo[k])
+    assert np.array_equal(v[i].view(np.uint32), lv[k].view(np.uint32))
+    assert np.array_equal(m[i], lm[k])
+    _feed(nm, ctx, gmap, None, frames[8:])
+    _feed(nm, ctx, loaded, None, frames[8:])
+    a, b = gmap.download(), loaded.download()
+    i = np.lexsort((a[0][:, 2], a[0][:, 1], a[0][:, 0]))
+    k = np.lexsort((b[0][:, 2], b[0][:, 1], b[0][:, 0]))
+    _same_map((a[0][i], a[1][i], a[2][i]), (b[0][k], b[1][k], b[2][k]))

@@ -213,6 +213,17 @@ def test_manager_maps_match_the_oracle(gpu, driver, tmp_path, filter_type,
     assert np.array_equal(go, oo)
     assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
     assert np.array_equal(gm, omk)
+    # the driver's map::Map::writeNanoVDB file (read back by the driver
+    # through the .nvdb constructor) holds the oracle's map
+    import nvdb_layout as L
+    with open(out + ".nvdb", "rb") as f:
+        _, grids = L.read_file(f.read())
+    leaves = grids[0][2].leaves()
+    no = np.array([x[0] for x in leaves], np.int32)
+    order = np.lexsort((no[:, 2], no[:, 1], no[:, 0]))
+    assert np.array_equal(no[order], oo)
+    nv = np.stack([x[1] for x in leaves])[order]
+    assert np.array_equal(nv.view(np.uint32), ov.view(np.uint32))
 
 
 @pytest.mark.gpu
@@ -326,6 +337,8 @@ def test_sim_manager_32_agents_match_a_manager_loop(gpu, driver, tmp_path):
                    cloud)
     snap = os.path.join(tmp, "hall.nmap")
     sim_map.save(snap)
+    nvdb = os.path.join(tmp, "hall.nvdb")
+    sim_map.write_nvdb(nvdb)
     o, v, _ = sim_map.download()
     sim_map.close()
     cfg = _write_config(tmp, 0.1, 0, [
@@ -343,6 +356,18 @@ def test_sim_manager_32_agents_match_a_manager_loop(gpu, driver, tmp_path):
     r = subprocess.run([driver, "sim", cfg, "2"], capture_output=True,
                        text=True, timeout=180)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the same grid given as a NanoVDB file ("SimGrid:" naming a .nvdb):
+    # the same agents, maps, collisions and spawn
+    with open(cfg) as f:
+        text = f.read()
+    cfg_nvdb = os.path.join(tmp, "config_nvdb.txt")
+    with open(cfg_nvdb, "w") as f:
+        f.write(text.replace(snap, nvdb))
+    print("SimManager: the grid from a .nvdb file", flush=True)
+    r2 = subprocess.run([driver, "sim", cfg_nvdb, "2"], capture_output=True,
+                        text=True, timeout=180)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert r2.stdout == r.stdout
     out = r.stdout.split("\n")
     summary = [l for l in out if l.startswith("agents ")][0].split()
     assert summary[1] == "32" and summary[-1] == "1"

@@ -1,7 +1,8 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU).
 
-tests/cpp/sanitize_main.cpp drives csrc/nm_scene.cpp, the oracle and the
-header-only Config/SensorData parser on small inputs.  It is built here with
+tests/cpp/sanitize_main.cpp drives csrc/nm_scene.cpp, csrc/nm_nvdb.cpp
+(NanoVDB files, damaged ones included), the oracle and the header-only
+Config/SensorData parser on small inputs.  It is built here with
 g++ (no GPU code; the oracle keeps its -frounding-math -ffp-contract=off
 flags) and run with every sanitizer report fatal.
 """
@@ -16,6 +17,7 @@ SRC = os.path.join(ROOT, "tests", "cpp", "sanitize_main.cpp")
 
 def _build(out):
     sources = [SRC, os.path.join(ROOT, "nanomap_amd", "csrc", "nm_scene.cpp"),
+               os.path.join(ROOT, "nanomap_amd", "csrc", "nm_nvdb.cpp"),
                os.path.join(ROOT, "oracle", "nm_oracle.cpp"),
                os.path.join(ROOT, "oracle", "nm_soft_oracle.cpp")]
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
