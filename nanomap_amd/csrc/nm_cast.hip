@@ -897,9 +897,14 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
       }
       while (t <= L) {
         // a miss on the current voxel (its exit time t <= L)
-        uint32_t mw = 0;
-        if (entry >= 0 && (entry & kSlotHasHits))
+        // in a leaf holding hits, also the voxel's run of sorted hits
+        // [s0, end), which the gap count below needs if the voxel has hits
+        uint32_t mw = 0, s0 = 0, end = 0;
+        if (entry >= 0 && (entry & kSlotHasHits)) {
           mw = hit_mask[cell >> 5];  // kMaskWords words per slot
+          s0 = hit_start[cell];
+          end = tiles[2 * (size_t)cell + 1];
+        }
         // the step: next crossing time of the axis (round-down: the wave's
         // FP64 mode is toward -inf here, so this is the rounded-down sum)
         const double dt = bx ? s.dx : (by ? s.dy : s.dz);
@@ -930,8 +935,6 @@ __global__ __launch_bounds__(kCastThreads) void k_accumulate(
             // its first later hit (the voxel's hits are sorted[s0, end) in
             // ascending point order); neighbouring rays mostly share the
             // gap, so these counts are merged in the table as well
-            const uint32_t s0 = hit_start[cell];
-            const uint32_t end = tiles[2 * (size_t)cell + 1];
             uint32_t lo = s0, hi = end;
             while (lo < hi) {
               const uint32_t mid = (lo + hi) >> 1;

@@ -253,7 +253,6 @@ constexpr int kMissRuns = 4096;
 // the hit sort, cleared with the entry by the update).
 constexpr int32_t kSlotHasHits = 1 << 30;
 constexpr int kUpdateGrid = 2048;
-constexpr int kRadixTile = 2048;
 // Voxels with more hits than Pipeline::heavy_hits in one launch sequence
 // are summed by a whole wavefront each, so that no lane sums a long
 // sequence alone (measured best from 16 hits on, for a single frame and for

@@ -25,15 +25,17 @@
 // use 8 iterations (2048 keys) for enough tiles to fill the GPU; large
 // ones up to 64, so the digit-major histogram (2^D words per tile, written
 // and read one word per digit row) stays well below the size of the keys.
+#include <algorithm>
+
 #include "nm_internal.h"
 
 namespace nm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMinItems = kRadixTile / kThreads;  // 8
 constexpr int kMaxItems = 64;
 constexpr int kWaves = kThreads / 64;
+constexpr int kMinItems = 8;
 constexpr int kChunk = 8;  // iterations whose loads are issued together
 static_assert(kMinItems % kChunk == 0, "items must be a multiple of kChunk");
 
@@ -234,7 +236,31 @@ void radix_pass(uint32_t* kin, uint32_t* vin, uint32_t* kout, uint32_t* vout,
 
 }  // namespace
 
-int radix_max_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
+// Iterations per tile for n keys: the smallest power of two from 8 to 64
+// that keeps the tiles at 2048 or fewer.  Small sorts use 8 (2048 keys a
+// tile; smaller tiles measured slower: each tile's prologue reads the 2^D
+// digit totals and its row of the histogram), large ones keep the
+// digit-major histogram (2^D words per tile) well below the size of the
+// keys.
+int radix_items(int n) {
+  int items = kMinItems;
+  while (items < kMaxItems && (int64_t)n > (int64_t)items * kThreads * 2048)
+    items *= 2;
+  return items;
+}
+int radix_blocks(int n) {
+  const int items = radix_items(n);
+  return (int)(((int64_t)n + (int64_t)items * kThreads - 1) / ((int64_t)items * kThreads));
+}
+// The most tiles any sort of up to n keys uses: at most 2048 while the
+// tiles can still grow, at most one per 64 iterations beyond.
+int radix_max_blocks(int n) {
+  const int64_t small = std::min<int64_t>(
+      2048, ((int64_t)n + kMinItems * kThreads - 1) / (kMinItems * kThreads));
+  const int64_t large = ((int64_t)n + (int64_t)kMaxItems * kThreads - 1) /
+                        ((int64_t)kMaxItems * kThreads);
+  return (int)std::max(small, large);
+}
 int radix_hist_words(int n) { return 2048 * (radix_max_blocks(n) + 2); }
 
 hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
@@ -243,11 +269,8 @@ hipError_t radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], int n,
   int cur = 0;
   *out_buf = 0;
   if (n <= 0) return hipSuccess;
-  // iterations per tile: at least 8, and enough for ~2048 tiles at most
-  int items = kMinItems;
-  while (items < kMaxItems && (int64_t)n > (int64_t)items * kThreads * 2048)
-    items *= 2;
-  const int nblocks = (n + items * kThreads - 1) / (items * kThreads);
+  const int items = radix_items(n);
+  const int nblocks = radix_blocks(n);
   const int passes = (bits + 10) / 11;
   const int D = (bits + passes - 1) / passes < 8 ? 8 : (bits + passes - 1) / passes;
   for (int shift = 0; shift < bits; shift += D) {
