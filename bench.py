@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # of the 1024 SIMDs at 2.4 GHz, in wave-instructions per second
 VALU_PEAK = 1024 * 2.4e9 / 4
 # wave64 VALU instructions per voxel visit of the accumulate pass on this
-# frame workload, from its SQ counters (profiles/k_accumulate_sq_counters.md)
-VALU_PER_VISIT = 2.15
+# frame workload (sliced walks included), from its SQ counters
+# (profiles/k_accumulate_sq_counters.md)
+VALU_PER_VISIT = 2.57
 
 
 def parse():
