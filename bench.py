@@ -88,7 +88,10 @@ def percentile(xs, q):
 def bench_batch(nm, W, torch, steps, rounding):
     """32 agents (uneven image sizes, ranges, voxel sizes; own maps): one
     graph-captured batched launch per pass, against the same frames as 32
-    plain launches one after another, on the same stream."""
+    plain launches one after another, on the same stream (`speedup`, the
+    target's baseline), and as 32 plain launches each waited for before the
+    next (`speedup_vs_sync_per_frame`, a per-frame call that returns once
+    its map is updated)."""
     _, agents = W.agents_32(nm)
     clouds = [torch.from_numpy(a["cloud"]).cuda() for a in agents]
     frames = [W.agent_frame(nm, a, rounding) for a in agents]
@@ -134,6 +137,16 @@ def bench_batch(nm, W, torch, steps, rounding):
             ctx.submit(m, f, c, stream=sh)
 
     plain_ms = timed(plain_pass)
+
+    def plain_sync_pass():
+        # each frame waited for before the next is submitted, as a call
+        # that returns once the map is updated (Manager::insertPointCloud,
+        # and the reference's agent loop, SimManager.h:100-113)
+        for f, c, m in zip(frames, clouds, pmaps):
+            ctx.submit(m, f, c, stream=sh)
+            stream.synchronize()
+
+    plain_sync_ms = timed(plain_sync_pass)
     batch_ms = timed(lambda: batch.submit(table, stream=sh))
     st = batch.stats()
     assert all(s["errors"] == 0 and s["overflow"] == 0 for s in st)
@@ -153,6 +166,8 @@ def bench_batch(nm, W, torch, steps, rounding):
         "plain_mrays_per_s": rays / plain_ms / 1e3,
         "batched_mrays_per_s": rays / batch_ms / 1e3,
         "speedup": plain_ms / batch_ms,
+        "plain_sync_ms": plain_sync_ms,
+        "speedup_vs_sync_per_frame": plain_sync_ms / batch_ms,
         "tuning": tuning,
         "tuner_s": tune_s,
         "steps": steps,

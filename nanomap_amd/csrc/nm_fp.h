@@ -9,7 +9,7 @@
 // sequences without restoring them.  So outside the two per-step DDA loops
 // round-down is emulated exactly here (those loops hold no divide or square
 // root; nm_cast.hip sets the wave's FP64 mode to toward -inf around them
-// and back to nearest after, and dda_step's kHwDown form then uses the
+// and back to nearest after, and their crossing-time adds are then the
 // hardware add):
 //   add: TwoSum error term; result stepped down one ulp if the error < 0;
 //   mul: FMA residual a*b - p; stepped down if < 0;
