@@ -557,8 +557,8 @@ def main():
     # Frame latency over the whole sequence (every frame once, from an
     # empty map, after 3 warm-up frames), whatever --steps is:
     #  - clouds resident in HBM: event pair around each frame's submit;
-    #  - as the baseline defines it, from the pinned host cloud to the map
-    #    updated on the device: the event pair also spans the H2D copy.
+    #  - from the pinned host cloud to the map updated on the device (the
+    #    headline latency): the event pair also spans the H2D copy.
     dcam = nm.depth_camera(cam.width, cam.height, cam.hfov_deg, cam.vfov_deg)
 
     def sequence_latency(host):

@@ -14,8 +14,8 @@ oracle is pinned by what the mathematics fixes:
       round-to-nearest), so a wrong tie rule, a dropped first voxel or an
       off-by-one end voxel fails;
   FP32 fold in a rounding mode: numpy sequential sums (nearest) and exact
-    rational rounding toward -inf, plus the bit patterns measured in the
-    survey (7 x -0.4f: 0xC0333334 nearest, 0xC0333335 down);
+    rational rounding toward -inf, plus known bit patterns (7 x -0.4f
+    summed in order: 0xC0333334 nearest, 0xC0333335 down);
   temp accumulate + integrate (Handler.cpp:88-122): closed forms for one
     ray, repeated frames, clamp saturation, activity thresholds;
   voxel filter: closed forms on exact inputs (integer eye: the endpoint
