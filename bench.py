@@ -107,13 +107,16 @@ def bench_batch(nm, W, torch, steps, rounding):
 
     stream = torch.cuda.current_stream()
     sh = ctypes_stream(stream)
-    ctx = nm.Context(P, T)
+    # tile capacity of both: 2^18 leaves, four times the 68.6 k the 32
+    # frames touch together (27-bit sort keys for both)
+    TILES = 1 << 18
+    ctx = nm.Context(P, T, tile_capacity=TILES)
     pmaps = maps()
     rays = 0
     for f, c, m in zip(frames, clouds, pmaps):  # untimed pass: ray counts
         ctx.submit(m, f, c, stream=sh)
         rays += ctx.stats()["rays"]
-    batch = nm.Batch(len(agents), total, T, 1 << 21)
+    batch = nm.Batch(len(agents), total, T, TILES)
     bmaps = maps()
     table = list(zip(frames, clouds, bmaps))
     t0 = time.perf_counter()
