@@ -433,9 +433,15 @@ def _near_wall_cloud(w, h, seed):
 
 
 @pytest.mark.parametrize("rounding", [RD, RN])
-def test_many_hit_voxels_bit_exact(gpu, rounding):
+@pytest.mark.parametrize("hit,miss", [(0.85, -0.4), (0.5, -0.25),
+                                      (0.001, -2.0), (1.0, -1e-20)])
+def test_many_hit_voxels_bit_exact(gpu, rounding, hit, miss):
     """Voxels with hundreds to thousands of hits in one frame, summed by the
-    wavefront-parallel path, against the oracle's sequential sums."""
+    wavefront-parallel path, against the oracle's sequential sums.  The
+    log-odds pairs cover a common fixed-point unit with steps of similar
+    size, power-of-two steps (round-to-nearest ties in every binade), steps
+    2^11 apart, and steps 2^66 apart (no common unit: the floating-point
+    path)."""
     nm = gpu
     w, h = 320, 240
     F = np.array([[0, 0, 1], [-1, 0, 0], [0, -1, 0]], np.float32)
@@ -444,7 +450,7 @@ def test_many_hit_voxels_bit_exact(gpu, rounding):
     omap = oracle.Map(*W.CLAMP)
     for k in range(2):
         f = nm.frame_from_pose([1.03 + 0.07 * k, 2.11, 0.97], [1, 0, 0, 0], F,
-                               0.2, 5.0, 0.85, -0.4, rounding, w, h)
+                               0.2, 5.0, hit, miss, rounding, w, h)
         temp = _run_frame(nm, ctx, gmap, omap, f, _near_wall_cloud(w, h, k))
         _, m, hh, _ = temp.export()
         print("hits per hit voxel: max %d, voxels > 256 hits: %d, mixed: %d"
