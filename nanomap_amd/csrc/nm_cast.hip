@@ -1523,44 +1523,56 @@ __global__ __launch_bounds__(256) void k_heavy(
 }
 
 // Folds the temp values into each tile's map leaf (Handler.cpp:88-122):
-// map += temp for nonzero temp values, clamp, activity thresholds.  One
-// workgroup per tile; clears the tile for the next frame.
+// map += temp for nonzero temp values, clamp, activity thresholds, and
+// clears the tile for the next frame.  One wavefront per tile, lane l
+// taking voxels l + 64 u (u < 8): the tile's eight rows and the map leaf's
+// are loaded together, and the leaf-wide decisions are ballots, so a
+// wavefront waits on one chain of memory latencies per tile.
 template <int M>
-__global__ __launch_bounds__(512) void k_apply(
+__global__ __launch_bounds__(256) void k_apply(
     const Job* __restrict__ jobs, const int32_t* __restrict__ slot_job,
     int32_t* __restrict__ leaf_slot, const int32_t* __restrict__ slot_leaf,
     uint32_t* __restrict__ tiles, uint32_t* __restrict__ hit_mask,
     const int32_t* __restrict__ map_idx, const int32_t* __restrict__ map_new,
     const MapView* __restrict__ maps, MapView single, int tile_cap,
     FrameCounters* __restrict__ cnt, uint32_t* __restrict__ slot_stats) {
-  __shared__ uint32_t s_part[8][3];
-  __shared__ int s_new;
-  const int li = threadIdx.x;
-  const int lane = li & 63, wave = li >> 6;
+  const int lane = threadIdx.x & 63;
   const int nleaves = min(cnt->n_leaves, tile_cap);
-  for (int slot = blockIdx.x; slot < nleaves; slot += gridDim.x) {
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  uint2* __restrict__ cells = reinterpret_cast<uint2*>(tiles);
+  for (int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       slot < nleaves; slot += nwaves) {
     const int jb = __builtin_amdgcn_readfirstlane(slot_job[slot]);
     const Job& J = jobs[jb];
     const MapView m = maps ? maps[J.map] : single;
-    const int c = slot_leaf[slot];
-    const size_t cell = (size_t)slot * 512 + li;
-    const uint32_t t0 = tiles[2 * cell];
-    const uint32_t t1 = tiles[2 * cell + 1];
-    const int known = map_idx[slot];
-    float old_v = 0.0f;
-    uint32_t old_w = 0;
-    if (known >= 0) {
-      old_v = m.values[(size_t)known * 512 + li];
-      old_w = m.masks[(size_t)known * kMaskWords + (li >> 5)];
+    const int c = __builtin_amdgcn_readfirstlane(slot_leaf[slot]);
+    const int known = __builtin_amdgcn_readfirstlane(map_idx[slot]);
+    uint2 t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = cells[(size_t)slot * 512 + u * 64 + lane];
+    float old_v[8];
+    uint32_t old_w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      old_v[u] = 0.0f;
+      old_w[u] = 0u;
     }
-    const bool touched = (t1 & kTouchedBit) != 0;
-    const uint32_t nhits = t1 & kHitsMask;
-    const float tv = touched ? float_of(t0) : 0.0f;
-    const bool nz = touched && tv != 0.0f;
-    const int any = __syncthreads_or(nz);
+    if (known >= 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        old_v[u] = m.values[(size_t)known * 512 + u * 64 + lane];
+        old_w[u] = m.masks[(size_t)known * kMaskWords + 2 * u + (lane >> 5)];
+      }
+    }
+    bool any_nz = false;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      any_nz |= (t[u].y & kTouchedBit) != 0 && float_of(t[u].x) != 0.0f;
+    const bool any = __ballot(any_nz) != 0;
     int idx = known;
     if (known < 0) {  // a leaf new to the map: its pool leaf is fresh (zero)
-      if (li == 0) {
+      int got = -1;
+      if (lane == 0) {
         const int r = map_new[slot];
         bool ok = false;
         if (r >= 0) {
@@ -1573,57 +1585,52 @@ __global__ __launch_bounds__(512) void k_apply(
             atomicAdd(m.leaf_count + 1, 1);
           }
         }
-        s_new = ok ? r : -1;
+        got = ok ? r : -1;
       }
-      __syncthreads();
-      idx = s_new;
+      idx = __builtin_amdgcn_readfirstlane(got);
     }
-    if (any && idx >= 0) {
-      float* mv = m.values + (size_t)idx * 512;
-      uint32_t* mm = m.masks + (size_t)idx * kMaskWords;
-      bool act = (old_w >> (li & 31)) & 1u;
-      if (nz) {
-        float v = add<M, true>(old_v, tv);
-        if (v > m.clamp_max) {
-          v = m.clamp_max;
-        } else if (v < m.clamp_min) {
-          v = m.clamp_min;
+    uint32_t n_touched = 0, n_hit = 0, n_mixed = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t t1 = t[u].y;
+      const bool touched = (t1 & kTouchedBit) != 0;
+      const uint32_t nhits = t1 & kHitsMask;
+      const float tv = touched ? float_of(t[u].x) : 0.0f;
+      const bool nz = touched && tv != 0.0f;
+      if (any && idx >= 0) {
+        bool act = (old_w[u] >> (lane & 31)) & 1u;
+        if (nz) {
+          float v = add<M, true>(old_v[u], tv);
+          if (v > m.clamp_max) {
+            v = m.clamp_max;
+          } else if (v < m.clamp_min) {
+            v = m.clamp_min;
+          }
+          if (v > m.hit_thr) {
+            act = true;
+          } else if (v < m.miss_thr) {
+            act = false;
+          }
+          m.values[(size_t)idx * 512 + u * 64 + lane] = v;
         }
-        if (v > m.hit_thr) {
-          act = true;
-        } else if (v < m.miss_thr) {
-          act = false;
-        }
-        mv[li] = v;
+        const uint64_t bits = __ballot(act);
+        if (lane < 2)
+          m.masks[(size_t)idx * kMaskWords + 2 * u + lane] =
+              (uint32_t)(lane == 0 ? bits : bits >> 32);
       }
-      const uint64_t b = __ballot(act);
-      if (lane == 0) {
-        mm[2 * wave] = (uint32_t)b;
-        mm[2 * wave + 1] = (uint32_t)(b >> 32);
-      }
+      if (touched) cells[(size_t)slot * 512 + u * 64 + lane] = make_uint2(0u, 0u);
+      // per-slot statistics: voxels touched, with hits, with hits and misses
+      n_touched += (uint32_t)__popcll(__ballot(touched));
+      n_hit += (uint32_t)__popcll(__ballot(nhits != 0));
+      n_mixed += (uint32_t)__popcll(__ballot(nhits != 0 && (t1 & kMixedBit)));
     }
-    if (touched) {
-      tiles[2 * cell] = 0;
-      tiles[2 * cell + 1] = 0;
-    }
-    if (li < kMaskWords) hit_mask[(size_t)slot * kMaskWords + li] = 0;
-    if (li == 0) leaf_slot[c] = -1;
-    // per-slot statistics: voxels touched, with hits, with hits and misses
-    const uint64_t bt = __ballot(touched);
-    const uint64_t bh = __ballot(nhits != 0);
-    const uint64_t bm = __ballot(nhits != 0 && (t1 & kMixedBit));
+    if (lane < kMaskWords) hit_mask[(size_t)slot * kMaskWords + lane] = 0;
     if (lane == 0) {
-      s_part[wave][0] = (uint32_t)__popcll(bt);
-      s_part[wave][1] = (uint32_t)__popcll(bh);
-      s_part[wave][2] = (uint32_t)__popcll(bm);
+      leaf_slot[c] = -1;
+      slot_stats[(size_t)slot * 4 + 0] = n_touched;
+      slot_stats[(size_t)slot * 4 + 1] = n_hit;
+      slot_stats[(size_t)slot * 4 + 2] = n_mixed;
     }
-    __syncthreads();
-    if (li < 3) {
-      uint32_t t = 0;
-      for (int w = 0; w < 8; ++w) t += s_part[w][li];
-      slot_stats[(size_t)slot * 4 + li] = t;
-    }
-    __syncthreads();
   }
 }
 
@@ -1903,6 +1910,7 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
   // serial chains
   const int hgrid = p.cu_count * 8;
   const int tgrid = p.cu_count * 8;
+  const int agrid = p.cu_count * 8;
   // With a side stream (a branch of the batch graph) the heavy voxels'
   // chains, which need only the cast's results, run beside the map lookup
   // and the per-voxel sums; both join before the apply pass.
@@ -1947,12 +1955,12 @@ hipError_t launch_update(const Pipeline& p, const Workspace& b,
     if (e != hipSuccess) return e;
   }
   if (down)
-    k_apply<kDown><<<kUpdateGrid, 512, 0, s>>>(
+    k_apply<kDown><<<agrid, 256, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
         b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
         b.slot_stats);
   else
-    k_apply<kNearest><<<kUpdateGrid, 512, 0, s>>>(
+    k_apply<kNearest><<<agrid, 256, 0, s>>>(
         b.jobs, b.slot_job, b.leaf_slot, b.slot_leaf, b.tiles, b.hit_mask,
         b.map_idx, b.map_new, map_table, single, p.tile_cap, b.counters,
         b.slot_stats);
