@@ -33,7 +33,7 @@
 //               the counts with nm::fold (exact, any rounding mode); a
 //               wavefront per voxel with many hits (k_heavy, on a forked
 //               stream beside the per-voxel pass).
-//  7 apply      one workgroup per tile: map += temp, clamp, activity
+//  7 apply      one wavefront per tile: map += temp, clamp, activity
 //               thresholds; tiles cleared.
 //
 // Counts are integers, so the result depends neither on the order in which
