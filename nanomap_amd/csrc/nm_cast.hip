@@ -1540,16 +1540,37 @@ __global__ __launch_bounds__(256) void k_apply(
   const int nleaves = min(cnt->n_leaves, tile_cap);
   const int nwaves = gridDim.x * (blockDim.x >> 6);
   uint2* __restrict__ cells = reinterpret_cast<uint2*>(tiles);
-  for (int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       slot < nleaves; slot += nwaves) {
-    const int jb = __builtin_amdgcn_readfirstlane(slot_job[slot]);
-    const Job& J = jobs[jb];
-    const MapView m = maps ? maps[J.map] : single;
-    const int c = __builtin_amdgcn_readfirstlane(slot_leaf[slot]);
-    const int known = __builtin_amdgcn_readfirstlane(map_idx[slot]);
+  // the next tile's rows and slot entries are requested before this
+  // tile's are folded
+  int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint2 tn[8];
+  int jbn = 0, cn = 0, knownn = -1;
+  if (slot < nleaves) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      tn[u] = cells[(size_t)slot * 512 + u * 64 + lane];
+    jbn = slot_job[slot];
+    cn = slot_leaf[slot];
+    knownn = map_idx[slot];
+  }
+  for (; slot < nleaves; slot += nwaves) {
     uint2 t[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = cells[(size_t)slot * 512 + u * 64 + lane];
+    for (int u = 0; u < 8; ++u) t[u] = tn[u];
+    const int jb = __builtin_amdgcn_readfirstlane(jbn);
+    const int c = __builtin_amdgcn_readfirstlane(cn);
+    const int known = __builtin_amdgcn_readfirstlane(knownn);
+    const int ns = slot + nwaves;
+    if (ns < nleaves) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        tn[u] = cells[(size_t)ns * 512 + u * 64 + lane];
+      jbn = slot_job[ns];
+      cn = slot_leaf[ns];
+      knownn = map_idx[ns];
+    }
+    const Job& J = jobs[jb];
+    const MapView m = maps ? maps[J.map] : single;
     float old_v[8];
     uint32_t old_w[8];
 #pragma unroll
